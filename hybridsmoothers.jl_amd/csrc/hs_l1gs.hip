@@ -1,0 +1,711 @@
+// hs_l1gs.hip -- partitioned l1 Gauss-Seidel (L1GS) on gfx950.
+//
+// Build: D_Dl1 and the in-partition strict triangles on the device
+// (reference: _precompute_and_pack_kernel!, src/l1_gauss_seidel.jl:1028-1066;
+// _diag_offpart :1101-1242; packing :1248-1519; DiagonalIndices :422-436).
+// Apply: forward, backward and symmetric sweeps, one partition per wave
+// segment (reference: _apply_sweep_kernel! :1068-1095, _apply_sweep! :846-906).
+//
+// Device layout of the packed triangles.  Block k (0-based) starts at
+// k*P*(P-1)/2 like the reference's packed buffers (:698-701), but inside a block
+// the triangles are stored COLUMN-major so that step j of the right-looking
+// recurrence reads one contiguous run of values, one per lane:
+//   L(i,j), j < i : boff + j*(Pa-1) - j*(j-1)/2 + (i-j-1)
+//   U(i,j), i < j : boff + j*(j-1)/2 + i
+// (Pa = rows of this partition; only the last partition can be shorter.)
+// The reference row-major layout is produced on request by hs_l1gs_get.
+//
+// Recurrence.  Lane i of a segment owns row i.  Forward step j: every lane
+// forms (x_i - acc_i)/d_i, the value of lane j is broadcast, and lanes i > j
+// add L(i,j)*y_j.  Row i therefore sums its terms in ascending j -- the order
+// of the reference's _accumulate_from_cache (:1547-1562) -- and the file is
+// compiled with -ffp-contract=off, so forward sweeps are bit-identical to the
+// reference arithmetic.  The backward recurrence adds in descending j.
+#include "hs_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+namespace hs {
+
+enum { MODE_FWD = 0, MODE_BWD = 1, MODE_SYM = 2 };
+
+// ---------------------------------------------------------------------------
+// layout helpers
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int64_t colL_off(int64_t j, int64_t Pa) {
+    return j * (Pa - 1) - (j * (j - 1)) / 2;
+}
+__host__ __device__ __forceinline__ int64_t colU_off(int64_t j) { return (j * (j - 1)) / 2; }
+
+__host__ __device__ __forceinline__ double dl1_rule(double a, double d, double eta) {
+    // _compute_D_Dl1_kernel! (:1016-1022): a_ii + (a_ii >= eta*d ? 0 : d/2)
+    double star = (a >= eta * d) ? 0.0 : d / 2.0;
+    return a + star;
+}
+
+// ---------------------------------------------------------------------------
+// setup kernels
+// ---------------------------------------------------------------------------
+// One thread per row g.  Reads row g of a 1-based CSR (or column g of a
+// symmetric CSC, which the reference reads as row g, :1218-1229), computes
+// D_Dl1[g] and scatters the in-partition strict entries of the row into the
+// column-major triangles.  want_L / want_U select the triangles to pack.
+template <class TS>
+__global__ void k_setup_rows(int64_t n, int64_t P, const int64_t *__restrict__ ptr,
+                             const int64_t *__restrict__ ind, const double *__restrict__ val,
+                             double eta, double *__restrict__ D, TS *__restrict__ Lc,
+                             TS *__restrict__ Uc, int want_L, int want_U) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int64_t k = g / P;
+    int64_t s = k * P;
+    int64_t e = (s + P < n ? s + P : n) - 1;
+    int64_t Pa = e - s + 1;
+    int64_t i = g - s;
+    int64_t boff = k * ((P * (P - 1)) / 2);
+    double a = 0.0, d = 0.0;
+    int64_t p0 = ptr[g] - 1, p1 = ptr[g + 1] - 1;
+    for (int64_t p = p0; p < p1; ++p) {
+        int64_t j = ind[p] - 1;
+        double v = val[p];
+        if (j == g) {
+            a = v;
+        } else if (j < s || j > e) {
+            d += fabs(v);
+        } else if (j < g) {
+            if (want_L) {
+                int64_t jl = j - s;
+                Lc[boff + colL_off(jl, Pa) + (i - jl - 1)] = Scalar<TS>::from_f64(v);
+            }
+        } else {
+            if (want_U) {
+                int64_t jl = j - s;
+                Uc[boff + colU_off(jl) + i] = Scalar<TS>::from_f64(v);
+            }
+        }
+    }
+    D[g] = dl1_rule(a, d, eta);
+}
+
+// One thread per column c of a non-symmetric 1-based CSC: packs the column's
+// in-partition entries (reference :1279-1324 lower, :1415-1469 upper).  A CSC
+// column c holds column c of both triangles, so lower writes are contiguous.
+template <class TS>
+__global__ void k_pack_cols(int64_t n, int64_t P, const int64_t *__restrict__ colptr,
+                            const int64_t *__restrict__ rowval, const double *__restrict__ val,
+                            TS *__restrict__ Lc, TS *__restrict__ Uc, int want_L, int want_U) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int64_t k = c / P;
+    int64_t s = k * P;
+    int64_t e = (s + P < n ? s + P : n) - 1;
+    int64_t Pa = e - s + 1;
+    int64_t jl = c - s;
+    int64_t boff = k * ((P * (P - 1)) / 2);
+    int64_t p0 = colptr[c] - 1, p1 = colptr[c + 1] - 1;
+    for (int64_t p = p0; p < p1; ++p) {
+        int64_t r = rowval[p] - 1;
+        if (r < s) continue;
+        if (r > e) break;
+        if (r > c) {
+            if (want_L) Lc[boff + colL_off(jl, Pa) + (r - c - 1)] = Scalar<TS>::from_f64(val[p]);
+        } else if (r < c) {
+            if (want_U) Uc[boff + colU_off(jl) + (r - s)] = Scalar<TS>::from_f64(val[p]);
+        }
+    }
+}
+
+// DiagonalIndices (:422-436): position of a_cc in column c by binary search;
+// the smallest failing column (missing or zero diagonal) is kept in *bad.
+__global__ void k_diag_indices(int64_t n, const int64_t *__restrict__ colptr,
+                               const int64_t *__restrict__ rowval, const double *__restrict__ val,
+                               int64_t *__restrict__ diag, unsigned long long *__restrict__ bad) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int64_t lo = colptr[c] - 1, hi = colptr[c + 1] - 1;  // [lo, hi)
+    int64_t end = hi;
+    int64_t want = c + 1;  // 1-based row index
+    while (lo < hi) {
+        int64_t mid = lo + ((hi - lo) >> 1);
+        if (rowval[mid] < want) lo = mid + 1; else hi = mid;
+    }
+    if (lo >= end || rowval[lo] != want || val[lo] == 0.0) {
+        atomicMin(bad, (unsigned long long)(c + 1));
+        diag[c] = -1;
+    } else {
+        diag[c] = lo;
+    }
+}
+
+template <class TD>
+__global__ void k_convert(int64_t n, const double *__restrict__ src, TD *__restrict__ dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (TD)src[i];
+}
+
+// ---------------------------------------------------------------------------
+// sweep kernels
+// ---------------------------------------------------------------------------
+template <int PM>
+__device__ __forceinline__ double seg_bcast(double v, int j) {
+    if constexpr (PM == 64) {
+        int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+        int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+        return __hiloint2double(hi, lo);
+    } else {
+        return __shfl(v, j, PM);
+    }
+}
+template <int PM>
+__device__ __forceinline__ float seg_bcast(float v, int j) {
+    if constexpr (PM == 64) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+    } else {
+        return __shfl(v, j, PM);
+    }
+}
+
+// One partition per PM-lane segment, 256 threads per workgroup.
+// TS: storage type of the triangles, TC: arithmetic type (double or float).
+// x and y may alias (each partition reads its rows before writing them).
+template <int PM, int MODE, class TS, class TC>
+__global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
+                                                   const TS *__restrict__ Lc,
+                                                   const TS *__restrict__ Uc,
+                                                   const TC *__restrict__ D, int64_t n, int P,
+                                                   int64_t nparts) {
+    constexpr int SEGS = 256 / PM;
+    const int lane = threadIdx.x % PM;
+    const int64_t k = (int64_t)blockIdx.x * SEGS + threadIdx.x / PM;
+    if (k >= nparts) return;
+    const int64_t s = k * (int64_t)P;
+    const int64_t rem = n - s;
+    const int Pa = (int)(rem < P ? rem : P);
+    const int64_t boff = k * (((int64_t)P * (P - 1)) / 2);
+    const bool live = lane < Pa;
+
+    TC xr = live ? (TC)x[s + lane] : (TC)0;
+    TC dr = live ? D[s + lane] : (TC)1;
+    TC yr = xr;
+
+    if constexpr (MODE != MODE_BWD) {
+        const TS *Lk = Lc + boff;
+        TC l[PM];
+#pragma unroll
+        for (int j = 0; j < PM; ++j) {
+            bool has = (lane > j) && live;
+            l[j] = has ? load_as<TC>(Lk[colL_off(j, Pa) + (lane - j - 1)]) : (TC)0;
+        }
+        TC acc = (TC)0;
+#pragma unroll
+        for (int j = 0; j < PM - 1; ++j) {
+            if (j < Pa - 1) {
+                TC yv = (xr - acc) / dr;
+                TC yj = seg_bcast<PM>(yv, j);
+                if (lane > j) acc = acc + l[j] * yj;
+            }
+        }
+        yr = (xr - acc) / dr;
+    }
+    if constexpr (MODE == MODE_SYM) {
+        xr = dr * yr;  // y .= D_Dl1 .* y  (:897)
+    }
+    if constexpr (MODE != MODE_FWD) {
+        const TS *Uk = Uc + boff;
+        TC u[PM];
+#pragma unroll
+        for (int j = 0; j < PM; ++j) {
+            bool has = (lane < j) && (j < Pa);
+            u[j] = has ? load_as<TC>(Uk[colU_off(j) + lane]) : (TC)0;
+        }
+        TC acc = (TC)0;
+#pragma unroll
+        for (int jj = 0; jj < PM - 1; ++jj) {
+            const int j = PM - 1 - jj;
+            if (j < Pa) {
+                TC yv = (xr - acc) / dr;
+                TC yj = seg_bcast<PM>(yv, j);
+                if (lane < j) acc = acc + u[j] * yj;
+            }
+        }
+        yr = (xr - acc) / dr;
+    }
+    if (live) y[s + lane] = (double)yr;
+}
+
+// Partitions larger than a wave: one thread per partition, rows in sequence,
+// summing in the reference's order (left-looking).  Correctness path for
+// large partsize (e.g. partsize = N in the Gauss-Seidel sanity checks).
+template <int MODE, class TS, class TC>
+__global__ void k_sweep_thread(const double *x, double *y, const TS *__restrict__ Lc,
+                               const TS *__restrict__ Uc, const TC *__restrict__ D, int64_t n,
+                               int64_t P, int64_t nparts) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nparts) return;
+    int64_t s = k * P;
+    int64_t Pa = (n - s < P) ? n - s : P;
+    int64_t boff = k * ((P * (P - 1)) / 2);
+    if (x != y)
+        for (int64_t i = 0; i < Pa; ++i) y[s + i] = x[s + i];
+    if (MODE != MODE_BWD) {
+        for (int64_t i = 0; i < Pa; ++i) {
+            TC acc = 0;
+            for (int64_t j = 0; j < i; ++j)
+                acc = acc + load_as<TC>(Lc[boff + colL_off(j, Pa) + (i - j - 1)]) * (TC)y[s + j];
+            y[s + i] = (double)(((TC)y[s + i] - acc) / D[s + i]);
+        }
+    }
+    if (MODE == MODE_SYM)
+        for (int64_t i = 0; i < Pa; ++i) y[s + i] = (double)(D[s + i] * (TC)y[s + i]);
+    if (MODE != MODE_FWD) {
+        for (int64_t i = Pa - 1; i >= 0; --i) {
+            TC acc = 0;
+            for (int64_t j = i + 1; j < Pa; ++j)
+                acc = acc + load_as<TC>(Uc[boff + colU_off(j) + i]) * (TC)y[s + j];
+            y[s + i] = (double)(((TC)y[s + i] - acc) / D[s + i]);
+        }
+    }
+}
+
+// MatrixViewCache: the in-partition entries are read straight from the matrix
+// (reference :1588-1737), one thread per partition as in the reference's
+// kernel.  rowwise=1: 1-based CSR or symmetric CSC read by rows;
+// rowwise=0: non-symmetric CSC with DiagonalIndices binary searches.
+__device__ __forceinline__ int64_t bsearch_row(const int64_t *rowval, int64_t lo, int64_t hi,
+                                               int64_t want) {
+    // 0-based positions, inclusive [lo, hi]; rowval is 1-based; want 1-based
+    int64_t left = lo, right = hi;
+    while (left <= right) {
+        int64_t mid = (left + right) >> 1;
+        int64_t r = rowval[mid];
+        if (r < want) left = mid + 1;
+        else if (r > want) right = mid - 1;
+        else return mid;
+    }
+    return -1;
+}
+
+__device__ double view_acc(int rowwise, int upper, const int64_t *ptr, const int64_t *ind,
+                           const double *val, const int64_t *diag, const double *y, int64_t g,
+                           int64_t s, int64_t e) {
+    double acc = 0.0;
+    if (rowwise) {
+        for (int64_t p = ptr[g] - 1; p < ptr[g + 1] - 1; ++p) {
+            int64_t j = ind[p] - 1;
+            if (!upper) {
+                if (s <= j && j < g) acc += val[p] * y[j];
+            } else {
+                if (g < j && j <= e) acc += val[p] * y[j];
+            }
+        }
+        return acc;
+    }
+    int64_t c0 = upper ? g + 1 : s;
+    int64_t c1 = upper ? e : (g - 1 < e ? g - 1 : e);
+    for (int64_t col = c0; col <= c1; ++col) {
+        int64_t di = diag[col];
+        int64_t lo = upper ? ptr[col] - 1 : di + 1;
+        int64_t hi = upper ? di - 1 : ptr[col + 1] - 2;
+        if (lo <= hi && ind[lo] <= g + 1 && ind[hi] >= g + 1) {
+            int64_t m = bsearch_row(ind, lo, hi, g + 1);
+            if (m >= 0) acc += val[m] * y[col];
+        }
+    }
+    return acc;
+}
+
+template <int MODE>
+__global__ void k_sweep_view(const double *x, double *y, const int64_t *__restrict__ ptr,
+                             const int64_t *__restrict__ ind, const double *__restrict__ val,
+                             const int64_t *__restrict__ diag, int rowwise,
+                             const double *__restrict__ D, int64_t n, int64_t P, int64_t nparts) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nparts) return;
+    int64_t s = k * P;
+    int64_t e = (s + P < n ? s + P : n) - 1;
+    if (x != y)
+        for (int64_t g = s; g <= e; ++g) y[g] = x[g];
+    if (MODE != MODE_BWD)
+        for (int64_t g = s; g <= e; ++g)
+            y[g] = (y[g] - view_acc(rowwise, 0, ptr, ind, val, diag, y, g, s, e)) / D[g];
+    if (MODE == MODE_SYM)
+        for (int64_t g = s; g <= e; ++g) y[g] = D[g] * y[g];
+    if (MODE != MODE_FWD)
+        for (int64_t g = e; g >= s; --g)
+            y[g] = (y[g] - view_acc(rowwise, 1, ptr, ind, val, diag, y, g, s, e)) / D[g];
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+struct L1GSPlan {
+    int64_t n = 0, P = 1, nparts = 0, nchunks = 1, chunksize = 1, S = 0;
+    int sweep = HS_SWEEP_SYMMETRIC, cache = HS_CACHE_PACKED, prec = HS_PREC_FP64;
+    int fmt = HS_FMT_CSC, sym = 0, rowwise = 1;
+    DBuf<double> D;   // D_Dl1 in FP64 (always kept: getter and FP64 sweeps)
+    DBuf<float> Df;   // FP32 copy for the reduced-precision sweeps
+    DBuf<uint8_t> L, U;  // packed triangles, element type by `prec`
+    // MatrixViewCache: device copy of the matrix (1-based, as given)
+    DBuf<int64_t> vptr, vind, vdiag;
+    DBuf<double> vval;
+    Staging stage;
+
+    size_t elem() const { return prec == HS_PREC_FP64 ? 8 : (prec == HS_PREC_FP32 ? 4 : 2); }
+};
+
+template <int PM, class TS, class TC>
+static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *D, hipStream_t st) {
+    constexpr int SEGS = 256 / PM;
+    unsigned grid = grid_for(h.nparts, SEGS);
+    const TS *L = reinterpret_cast<const TS *>(h.L.get());
+    const TS *U = reinterpret_cast<const TS *>(h.U.get());
+    int P = (int)h.P;
+    switch (h.sweep) {
+    case HS_SWEEP_FORWARD:
+        hipLaunchKernelGGL((k_sweep_seg<PM, MODE_FWD, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
+                           L, U, D, h.n, P, h.nparts);
+        break;
+    case HS_SWEEP_BACKWARD:
+        hipLaunchKernelGGL((k_sweep_seg<PM, MODE_BWD, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
+                           L, U, D, h.n, P, h.nparts);
+        break;
+    default:
+        hipLaunchKernelGGL((k_sweep_seg<PM, MODE_SYM, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
+                           L, U, D, h.n, P, h.nparts);
+        break;
+    }
+}
+
+template <class TS, class TC>
+static void launch_packed(const L1GSPlan &h, const double *x, double *y, const TC *D,
+                          hipStream_t st) {
+    if (h.P <= 4) launch_seg<4, TS, TC>(h, x, y, D, st);
+    else if (h.P <= 8) launch_seg<8, TS, TC>(h, x, y, D, st);
+    else if (h.P <= 16) launch_seg<16, TS, TC>(h, x, y, D, st);
+    else if (h.P <= 32) launch_seg<32, TS, TC>(h, x, y, D, st);
+    else if (h.P <= 64) launch_seg<64, TS, TC>(h, x, y, D, st);
+    else {
+        const TS *L = reinterpret_cast<const TS *>(h.L.get());
+        const TS *U = reinterpret_cast<const TS *>(h.U.get());
+        unsigned grid = grid_for(h.nparts, 64);
+        if (h.sweep == HS_SWEEP_FORWARD)
+            hipLaunchKernelGGL((k_sweep_thread<MODE_FWD, TS, TC>), dim3(grid), dim3(64), 0, st, x,
+                               y, L, U, D, h.n, h.P, h.nparts);
+        else if (h.sweep == HS_SWEEP_BACKWARD)
+            hipLaunchKernelGGL((k_sweep_thread<MODE_BWD, TS, TC>), dim3(grid), dim3(64), 0, st, x,
+                               y, L, U, D, h.n, h.P, h.nparts);
+        else
+            hipLaunchKernelGGL((k_sweep_thread<MODE_SYM, TS, TC>), dim3(grid), dim3(64), 0, st, x,
+                               y, L, U, D, h.n, h.P, h.nparts);
+    }
+}
+
+// Device-side application y = M^-1 x (x, y device pointers; may alias).
+void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_t st) {
+    if (h.n == 0) return;
+    if (h.cache == HS_CACHE_VIEW) {
+        unsigned grid = grid_for(h.nparts, 64);
+        const int64_t *dg = h.vdiag.get();
+        if (h.sweep == HS_SWEEP_FORWARD)
+            hipLaunchKernelGGL(k_sweep_view<MODE_FWD>, dim3(grid), dim3(64), 0, st, x, y,
+                               h.vptr.get(), h.vind.get(), h.vval.get(), dg, h.rowwise, h.D.get(),
+                               h.n, h.P, h.nparts);
+        else if (h.sweep == HS_SWEEP_BACKWARD)
+            hipLaunchKernelGGL(k_sweep_view<MODE_BWD>, dim3(grid), dim3(64), 0, st, x, y,
+                               h.vptr.get(), h.vind.get(), h.vval.get(), dg, h.rowwise, h.D.get(),
+                               h.n, h.P, h.nparts);
+        else
+            hipLaunchKernelGGL(k_sweep_view<MODE_SYM>, dim3(grid), dim3(64), 0, st, x, y,
+                               h.vptr.get(), h.vind.get(), h.vval.get(), dg, h.rowwise, h.D.get(),
+                               h.n, h.P, h.nparts);
+    } else if (h.prec == HS_PREC_FP64) {
+        launch_packed<double, double>(h, x, y, h.D.get(), st);
+    } else if (h.prec == HS_PREC_FP32) {
+        launch_packed<float, float>(h, x, y, h.Df.get(), st);
+    } else {
+        launch_packed<__hip_bfloat16, float>(h, x, y, h.Df.get(), st);
+    }
+    check_launch();
+}
+
+// D_Dl1 of a non-symmetric CSC: the reference scans every column for row i
+// (:1135-1216), visiting row i's entries in ascending column order.  The same
+// sums come from the rows of the CSR transpose (a stable counting sort keeps
+// ascending columns inside each row), in O(nnz) instead of O(N^2).
+static void nonsym_D(L1GSPlan &h, const int64_t *h_ptr, const int64_t *h_ind, const double *h_val,
+                     double eta, hipStream_t st) {
+    const int64_t n = h.n;
+    const int64_t nnz = h_ptr[n] - 1;
+    std::vector<int64_t> rp(n + 1, 0), ci(nnz > 0 ? nnz : 1);
+    std::vector<double> cv(nnz > 0 ? nnz : 1);
+    for (int64_t p = 0; p < nnz; ++p) rp[h_ind[p]] += 1;  // 1-based row r counted in rp[r]
+    rp[0] = 1;
+    for (int64_t i = 1; i <= n; ++i) rp[i] += rp[i - 1];
+    {
+        std::vector<int64_t> cur(rp.begin(), rp.end() - 1);
+        for (int64_t col = 1; col <= n; ++col)
+            for (int64_t p = h_ptr[col - 1]; p <= h_ptr[col] - 1; ++p) {
+                int64_t r = h_ind[p - 1];
+                int64_t q = cur[r - 1]++;
+                ci[q - 1] = col;
+                cv[q - 1] = h_val[p - 1];
+            }
+    }
+    DBuf<int64_t> trp(n + 1), tci(nnz);
+    DBuf<double> tcv(nnz);
+    HS_CHECK_HIP(hipMemcpyAsync(trp.get(), rp.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
+    if (nnz) {
+        HS_CHECK_HIP(hipMemcpyAsync(tci.get(), ci.data(), nnz * 8, hipMemcpyHostToDevice, st));
+        HS_CHECK_HIP(hipMemcpyAsync(tcv.get(), cv.data(), nnz * 8, hipMemcpyHostToDevice, st));
+    }
+    hipLaunchKernelGGL((k_setup_rows<double>), dim3(grid_for(n, 256)), dim3(256), 0, st, n, h.P,
+                       trp.get(), tci.get(), tcv.get(), eta, h.D.get(), (double *)nullptr,
+                       (double *)nullptr, 0, 0);
+    check_launch();
+    HS_CHECK_HIP(hipStreamSynchronize(st));  // the host arrays above go out of scope
+}
+
+template <class TS>
+static void build_packed(L1GSPlan &h, const DBuf<int64_t> &ptr, const DBuf<int64_t> &ind,
+                         const DBuf<double> &val, const int64_t *h_ptr, const int64_t *h_ind,
+                         const double *h_val, double eta, hipStream_t st) {
+    const int want_L = h.sweep != HS_SWEEP_BACKWARD;
+    const int want_U = h.sweep != HS_SWEEP_FORWARD;
+    if (want_L) {
+        h.L.alloc(h.S * sizeof(TS));
+        HS_CHECK_HIP(hipMemsetAsync(h.L.get(), 0, h.L.bytes(), st));
+    }
+    if (want_U) {
+        h.U.alloc(h.S * sizeof(TS));
+        HS_CHECK_HIP(hipMemsetAsync(h.U.get(), 0, h.U.bytes(), st));
+    }
+    TS *L = reinterpret_cast<TS *>(h.L.get());
+    TS *U = reinterpret_cast<TS *>(h.U.get());
+    unsigned grid = grid_for(h.n, 256);
+    if (h.rowwise) {
+        // one pass: D_Dl1 and both triangles (the reference launches its
+        // packing kernel once per triangle and recomputes D_Dl1, :805-837)
+        hipLaunchKernelGGL((k_setup_rows<TS>), dim3(grid), dim3(256), 0, st, h.n, h.P, ptr.get(),
+                           ind.get(), val.get(), eta, h.D.get(), L, U, want_L, want_U);
+        check_launch();
+        return;
+    }
+    nonsym_D(h, h_ptr, h_ind, h_val, eta, st);
+    hipLaunchKernelGGL((k_pack_cols<TS>), dim3(grid), dim3(256), 0, st, h.n, h.P, ptr.get(),
+                       ind.get(), val.get(), L, U, want_L, want_U);
+    check_launch();
+}
+
+}  // namespace hs
+
+struct hs_l1gs : hs::L1GSPlan {};
+
+using namespace hs;
+
+extern "C" {
+
+int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const double *val, int fmt,
+                  int is_sym, int64_t partsize, double eta, int sweep, int cache, int precision,
+                  int64_t threads, int64_t blocks, void *stream, hs_l1gs_t *out) {
+    return guard([&] {
+        HS_REQUIRE(out != nullptr, HS_EINVAL, "out handle pointer is NULL");
+        *out = nullptr;
+        HS_REQUIRE(partsize > 0, HS_EINVAL, "partsize must be greater than 0");
+        HS_REQUIRE(threads > 0, HS_EINVAL, "`threads` must be greater than 0");
+        HS_REQUIRE(blocks > 0, HS_EINVAL, "`blocks` must be greater than 0");
+        HS_REQUIRE(n >= 0, HS_EINVAL, "matrix size must be non-negative");
+        HS_REQUIRE(fmt == HS_FMT_CSC || fmt == HS_FMT_CSR, HS_EINVAL, "unknown matrix format");
+        HS_REQUIRE(sweep >= 0 && sweep <= 2, HS_EINVAL, "unknown sweep");
+        HS_REQUIRE(cache == HS_CACHE_PACKED || cache == HS_CACHE_VIEW, HS_EINVAL,
+                   "unknown cache strategy");
+        HS_REQUIRE(precision >= HS_PREC_FP64 && precision <= HS_PREC_BF16, HS_EINVAL,
+                   "unknown precision");
+        HS_REQUIRE(!(cache == HS_CACHE_VIEW && precision != HS_PREC_FP64), HS_EUNSUPPORTED,
+                   "MatrixViewCache is provided in FP64 only");
+        HS_REQUIRE(n == 0 || (ptr && ind && val), HS_EINVAL, "matrix arrays are NULL");
+        hipStream_t st = as_stream(stream);
+        std::unique_ptr<hs_l1gs> h(new hs_l1gs());
+        h->n = n;
+        h->P = partsize;
+        h->sweep = sweep;
+        h->cache = cache;
+        h->prec = precision;
+        h->fmt = fmt;
+        h->sym = is_sym ? 1 : 0;
+        h->rowwise = (fmt == HS_FMT_CSR) || is_sym;
+        // _blockpartitioning, GPU (:353-365)
+        h->nchunks = blocks;
+        h->nparts = (int64_t)std::ceil((double)n / (double)partsize);
+        int64_t cs = (int64_t)std::ceil((double)h->nparts / (double)blocks);
+        h->chunksize = cs <= threads ? cs : threads;
+        int64_t last = n - (h->nparts - 1) * partsize;
+        h->S = (partsize * (partsize - 1) * (h->nparts - 1)) / 2 + (last * (last - 1)) / 2;
+        if (n == 0) h->S = 0;
+
+        const int64_t nnz = n > 0 ? ptr[n] - 1 : 0;
+        HS_REQUIRE(nnz >= 0, HS_EINVAL, "invalid pointer array");
+        DBuf<int64_t> dptr(n + 1), dind(nnz);
+        DBuf<double> dval(nnz);
+        if (n > 0) {
+            HS_CHECK_HIP(hipMemcpyAsync(dptr.get(), ptr, (n + 1) * 8, hipMemcpyHostToDevice, st));
+            if (nnz) {
+                HS_CHECK_HIP(hipMemcpyAsync(dind.get(), ind, nnz * 8, hipMemcpyHostToDevice, st));
+                HS_CHECK_HIP(hipMemcpyAsync(dval.get(), val, nnz * 8, hipMemcpyHostToDevice, st));
+            }
+        }
+        DBuf<int64_t> ddiag;
+        if (!h->rowwise && n > 0) {
+            // DiagonalIndices: SingularException(col) on a missing or zero diagonal
+            ddiag.alloc(n);
+            DBuf<unsigned long long> bad(1);
+            unsigned long long init = ~0ull, got = 0;
+            HS_CHECK_HIP(hipMemcpyAsync(bad.get(), &init, 8, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_diag_indices, dim3(grid_for(n, 256)), dim3(256), 0, st, n,
+                               dptr.get(), dind.get(), dval.get(), ddiag.get(), bad.get());
+            check_launch();
+            HS_CHECK_HIP(hipMemcpyAsync(&got, bad.get(), 8, hipMemcpyDeviceToHost, st));
+            HS_CHECK_HIP(hipStreamSynchronize(st));
+            if (got != ~0ull) {
+                set_error(HS_ESINGULAR, "SingularException(" + std::to_string(got) + ")",
+                          (int64_t)got);
+                throw Error{HS_ESINGULAR};
+            }
+        }
+        h->D.alloc(n);
+        if (n > 0) {
+            if (cache == HS_CACHE_PACKED) {
+                if (precision == HS_PREC_FP64)
+                    build_packed<double>(*h, dptr, dind, dval, ptr, ind, val, eta, st);
+                else if (precision == HS_PREC_FP32)
+                    build_packed<float>(*h, dptr, dind, dval, ptr, ind, val, eta, st);
+                else
+                    build_packed<__hip_bfloat16>(*h, dptr, dind, dval, ptr, ind, val, eta, st);
+            } else {
+                // MatrixViewCache: D_Dl1 only; keep the matrix on the device
+                if (h->rowwise) {
+                    hipLaunchKernelGGL((k_setup_rows<double>), dim3(grid_for(n, 256)), dim3(256),
+                                       0, st, n, h->P, dptr.get(), dind.get(), dval.get(), eta,
+                                       h->D.get(), (double *)nullptr, (double *)nullptr, 0, 0);
+                    check_launch();
+                } else {
+                    nonsym_D(*h, ptr, ind, val, eta, st);
+                }
+                // the view kernels read 0-based diagonal positions
+                h->vptr = std::move(dptr);
+                h->vind = std::move(dind);
+                h->vval = std::move(dval);
+                h->vdiag = std::move(ddiag);
+            }
+            if (precision != HS_PREC_FP64) {
+                h->Df.alloc(n);
+                hipLaunchKernelGGL((k_convert<float>), dim3(grid_for(n, 256)), dim3(256), 0, st, n,
+                                   h->D.get(), h->Df.get());
+                check_launch();
+            }
+        }
+        HS_CHECK_HIP(hipStreamSynchronize(st));
+        *out = h.release();
+    });
+}
+
+int hs_l1gs_apply(hs_l1gs_t h, double *y, const double *x, int loc, void *stream, int flags) {
+    return guard([&] {
+        HS_REQUIRE(h != nullptr, HS_EINVAL, "NULL preconditioner handle");
+        HS_REQUIRE(h->n == 0 || (x && y), HS_EINVAL, "NULL vector");
+        hipStream_t st = as_stream(stream);
+        if (h->n == 0) return;
+        if (loc == HS_LOC_DEVICE) {
+            l1gs_apply_device(*h, x, y, st);
+            finish(st, flags);
+        } else {
+            // host vectors: y .= x; adapt; sweep; copyto!(y, _y)  (:367-384)
+            size_t bytes = (size_t)h->n * 8;
+            h->stage.ensure(h->n);
+            HS_CHECK_HIP(hipMemcpyAsync(h->stage.a.get(), x, bytes, hipMemcpyHostToDevice, st));
+            l1gs_apply_device(*h, h->stage.a.get(), h->stage.a.get(), st);
+            HS_CHECK_HIP(hipMemcpyAsync(y, h->stage.a.get(), bytes, hipMemcpyDeviceToHost, st));
+            HS_CHECK_HIP(hipStreamSynchronize(st));
+        }
+    });
+}
+
+int hs_l1gs_info(hs_l1gs_t h, int64_t *out) {
+    return guard([&] {
+        HS_REQUIRE(h && out, HS_EINVAL, "NULL argument");
+        out[0] = h->P;
+        out[1] = h->nparts;
+        out[2] = h->nchunks;
+        out[3] = h->chunksize;
+        out[4] = h->n;
+        out[5] = h->S;
+        out[6] = h->sweep;
+        out[7] = h->cache;
+    });
+}
+
+int hs_l1gs_get(hs_l1gs_t h, int what, double *out) {
+    return guard([&] {
+        HS_REQUIRE(h && out, HS_EINVAL, "NULL argument");
+        const int64_t n = h->n, P = h->P;
+        if (what == HS_GET_D_DL1) {
+            if (n) HS_CHECK_HIP(hipMemcpy(out, h->D.get(), n * 8, hipMemcpyDeviceToHost));
+            return;
+        }
+        HS_REQUIRE(what == HS_GET_SL || what == HS_GET_SU, HS_EINVAL, "unknown field");
+        HS_REQUIRE(h->cache == HS_CACHE_PACKED, HS_EINVAL,
+                   "packed buffers exist only for PackedBufferCache");
+        const DBuf<uint8_t> &buf = (what == HS_GET_SL) ? h->L : h->U;
+        const bool has = (what == HS_GET_SL) ? h->sweep != HS_SWEEP_BACKWARD
+                                             : h->sweep != HS_SWEEP_FORWARD;
+        HS_REQUIRE(has, HS_EINVAL,
+                   what == HS_GET_SL ? "this sweep has no lower buffer"
+                                     : "this sweep has no upper buffer");
+        std::vector<double> internal(h->S);
+        const size_t es = h->elem();
+        std::vector<uint8_t> raw(h->S * es);
+        if (h->S) HS_CHECK_HIP(hipMemcpy(raw.data(), buf.get(), raw.size(), hipMemcpyDeviceToHost));
+        for (int64_t t = 0; t < h->S; ++t) {
+            if (h->prec == HS_PREC_FP64) {
+                double v;
+                std::memcpy(&v, &raw[t * 8], 8);
+                internal[t] = v;
+            } else if (h->prec == HS_PREC_FP32) {
+                float v;
+                std::memcpy(&v, &raw[t * 4], 4);
+                internal[t] = v;
+            } else {
+                uint16_t b;
+                std::memcpy(&b, &raw[t * 2], 2);
+                uint32_t w = (uint32_t)b << 16;
+                float v;
+                std::memcpy(&v, &w, 4);
+                internal[t] = v;
+            }
+        }
+        // column-major (device) -> row-major (reference :1264, :1398)
+        for (int64_t k = 0; k < h->nparts; ++k) {
+            int64_t s = k * P;
+            int64_t Pa = (n - s < P) ? n - s : P;
+            int64_t boff = k * ((P * (P - 1)) / 2);
+            for (int64_t i = 0; i < Pa; ++i) {
+                if (what == HS_GET_SL) {
+                    for (int64_t j = 0; j < i; ++j)
+                        out[boff + (i * (i - 1)) / 2 + j] = internal[boff + colL_off(j, Pa) + (i - j - 1)];
+                } else {
+                    int64_t row_start = i * Pa - (i * (i + 1)) / 2;
+                    for (int64_t j = i + 1; j < Pa; ++j)
+                        out[boff + row_start + (j - i - 1)] = internal[boff + colU_off(j) + i];
+                }
+            }
+        }
+    });
+}
+
+int hs_l1gs_destroy(hs_l1gs_t h) {
+    return guard([&] { delete h; });
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+/*
+ * hsmooth.h -- C ABI of libhsmooth.so, the MI355X (gfx950) smoother library.
+ *
+ * Plain C types only (int64_t, double, void*).  Sparse inputs are host arrays
+ * in the layout Julia's SparseMatrixCSC{Float64,Int64} / SparseMatrixCSR holds:
+ * 1-based Int64 pointer and index arrays, Float64 values.  Vector arguments
+ * are host or device pointers as selected by `loc`.  Every call is synchronous
+ * unless HS_FLAG_ASYNC is passed, matching the reference's `synchronize` after
+ * each launch.  On failure a negative status is returned and hs_last_error()
+ * describes it; HS_ESINGULAR stores the offending 1-based column in
+ * hs_last_error_aux() (Julia maps it to LinearAlgebra.SingularException(col)).
+ *
+ * Reference interfaces replaced (HybridSmoothers.jl v0.1.4, file:line):
+ *   hs_l1gs_build   -> (builder::L1GSPrecBuilder)(A, partsize; isSymA, eta, sweep,
+ *                      cache_strategy)  src/l1_gauss_seidel.jl:287-338, with
+ *                      _blockpartitioning (GPU) :353-365, DiagonalIndices :422-436,
+ *                      _make_sweep_plan :626-843, _precompute_and_pack_kernel! :1028-1066
+ *   hs_l1gs_apply   -> LinearSolve.ldiv!(y, P, x)  :367-384 (+ _apply_sweep! :846-906)
+ *   hs_l1gs_info    -> P.partitioning fields  :118-124
+ *   hs_l1gs_get     -> P.sweep.{op,lop,uop}.D_DL1, .L.SLbuffer, .U.SUbuffer  :487-518
+ *   hs_cheb_build   -> _build_smoother(ChebyshevFourth|ChebyshevFirst, A)
+ *                      src/chebyshev.jl:141-156, :258-276
+ *   hs_cheb_smooth  -> smooth!(x, s, b)  src/chebyshev.jl:158-183, :278-322
+ *   hs_cheb_get     -> s.l1inv, s.rho, s.a, s.r, s.z, s.tmp  src/chebyshev.jl:121-129, :224-234
+ *   hs_csr_*        -> the SparseMatrixCSR operator of chebyshev.jl:144 / the
+ *                      LinearProblem matrix the Krylov solver multiplies by
+ *   hs_pcg_solve    -> solve(LinearProblem(A, b), KrylovJL_CG(); Pl = P)
+ *                      (README.md:27-29, test/test_preconditioners.jl:93)
+ */
+#ifndef HSMOOTH_H
+#define HSMOOTH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define HS_OK 0
+#define HS_EINVAL (-1)         /* bad argument: Julia ErrorException */
+#define HS_ESINGULAR (-2)      /* missing/zero diagonal: SingularException(aux) */
+#define HS_EDEVICE (-3)        /* HIP runtime error */
+#define HS_ENOMEM (-4)         /* device allocation failed */
+#define HS_ENODEVICE (-5)      /* backend not functional (no usable GPU) */
+#define HS_EUNSUPPORTED (-6)   /* combination not provided by this library */
+
+/* enums */
+#define HS_FMT_CSC 0
+#define HS_FMT_CSR 1
+#define HS_SWEEP_FORWARD 0
+#define HS_SWEEP_BACKWARD 1
+#define HS_SWEEP_SYMMETRIC 2
+#define HS_CACHE_PACKED 0      /* PackedBufferCache */
+#define HS_CACHE_VIEW 1        /* MatrixViewCache */
+#define HS_PREC_FP64 0         /* FP64 values, FP64 arithmetic */
+#define HS_PREC_FP32 1         /* FP32 values and packed buffers, FP32 accumulation */
+#define HS_PREC_BF16 2         /* BF16 off-diagonals, FP32 diagonal and accumulation */
+#define HS_LOC_HOST 0
+#define HS_LOC_DEVICE 1
+#define HS_FLAG_ASYNC 1        /* do not synchronise the stream before returning */
+
+#define HS_GET_D_DL1 0
+#define HS_GET_SL 1            /* strict lower, reference packed layout */
+#define HS_GET_SU 2            /* strict upper, reference packed layout */
+
+typedef struct hs_l1gs *hs_l1gs_t;
+typedef struct hs_csr *hs_csr_t;
+typedef struct hs_cheb *hs_cheb_t;
+
+/* ---- library ---------------------------------------------------------- */
+/* Selects the device and checks that it is a usable GPU (KA `functional`). */
+int hs_init(int device);
+const char *hs_last_error(void);
+int64_t hs_last_error_aux(void);
+/* Bytes of device memory held by live handles of this process. */
+int64_t hs_device_bytes(void);
+
+/* ---- L1 Gauss-Seidel -------------------------------------------------- */
+/*
+ * Builds the preconditioner on the device.  n: rows (square matrix);
+ * ptr/ind/val: 1-based CSC (fmt=HS_FMT_CSC) or CSR (HS_FMT_CSR) host arrays;
+ * is_sym: isSymA (ignored for CSR, whose rows are read directly);
+ * threads/blocks: GPUConfig, used for the partitioning fields exactly as
+ * _blockpartitioning does (results never depend on them).
+ */
+int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const double *val,
+                  int fmt, int is_sym, int64_t partsize, double eta, int sweep, int cache,
+                  int precision, int64_t threads, int64_t blocks, void *stream,
+                  hs_l1gs_t *out);
+/* y = M^-1 x.  x and y may alias.  loc: HS_LOC_HOST or HS_LOC_DEVICE. */
+int hs_l1gs_apply(hs_l1gs_t h, double *y, const double *x, int loc, void *stream, int flags);
+/* out[0..7] = partsize, nparts, nchunks, chunksize, n, packed_size, sweep, cache */
+int hs_l1gs_info(hs_l1gs_t h, int64_t *out);
+/* Copies D_DL1 (n) or SL/SU (packed_size) in the reference layout to host `out`. */
+int hs_l1gs_get(hs_l1gs_t h, int what, double *out);
+int hs_l1gs_destroy(hs_l1gs_t h);
+
+/* ---- device sparse operator (CSR) ------------------------------------- */
+/* Device CSR of A from 1-based CSC or CSR host arrays (CSC is transposed). */
+int hs_csr_create(int64_t n, const int64_t *ptr, const int64_t *ind, const double *val,
+                  int fmt, void *stream, hs_csr_t *out);
+/* y = A x on device or host vectors. */
+int hs_csr_spmv(hs_csr_t A, double *y, const double *x, int loc, void *stream, int flags);
+/* out[0..1] = n, nnz */
+int hs_csr_info(hs_csr_t A, int64_t *out);
+/* Copies 1-based rowptr (n+1), colval (nnz) and nzval (nnz) to host arrays. */
+int hs_csr_get(hs_csr_t A, int64_t *rowptr, int64_t *colval, double *nzval);
+int hs_csr_destroy(hs_csr_t A);
+
+/* ---- Chebyshev smoothers --------------------------------------------- */
+/* kind: 4 (ChebyshevFourth) or 1 (ChebyshevFirst).  Power iteration in FP64
+ * on the device.  precision: HS_PREC_FP64 or HS_PREC_FP32 (values of the
+ * smoother's SpMV and recurrences; the spectral estimate stays FP64). */
+int hs_cheb_build(int64_t n, const int64_t *colptr, const int64_t *rowval, const double *nzval,
+                  int kind, int degree, int power_iters, int precision, void *stream,
+                  hs_cheb_t *out);
+/* smooth!(x, s, b): x updated in place. */
+int hs_cheb_smooth(hs_cheb_t s, double *x, const double *b, int loc, void *stream, int flags);
+/* z = 0; smooth!(z, s, r)  -- the smoother used as a left preconditioner. */
+int hs_cheb_apply(hs_cheb_t s, double *z, const double *r, int loc, void *stream, int flags);
+/* out = {rho, a*, degree, kind, n} */
+int hs_cheb_info(hs_cheb_t s, double *out);
+/* what: 0 l1inv, 1 r, 2 z, 3 tmp (n doubles each) */
+int hs_cheb_get(hs_cheb_t s, int what, double *out);
+hs_csr_t hs_cheb_matrix(hs_cheb_t s);
+int hs_cheb_destroy(hs_cheb_t s);
+
+/* ---- preconditioned CG ------------------------------------------------ */
+#define HS_PCG_NONE 0
+#define HS_PCG_L1GS 1
+#define HS_PCG_CHEB 2
+
+typedef struct {
+    int64_t iters;        /* CG iterations taken */
+    int64_t status;       /* 0 solved, 1 iteration limit, 2 breakdown (p'Ap <= 0) */
+    double rnorm0;        /* sqrt(r0' z0) */
+    double rnorm;         /* final sqrt(r' z) */
+    double true_relres;   /* ||b - A x|| / ||b||, FP64 on the device */
+    double seconds;       /* wall time of the solve on the host clock */
+} hs_pcg_stats;
+
+/*
+ * CG with x0 = 0 and left preconditioner `prec` (an hs_l1gs_t or hs_cheb_t,
+ * as given by prec_kind), stopping when sqrt(r'z) <= atol + rtol*sqrt(r0'z0)
+ * (Krylov.jl `cg` rule) or after itmax iterations.  FP64 throughout the outer
+ * iteration whatever the preconditioner's precision.
+ */
+int hs_pcg_solve(hs_csr_t A, int prec_kind, void *prec, const double *b, double *x,
+                 double rtol, double atol, int64_t itmax, int loc, void *stream,
+                 hs_pcg_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HSMOOTH_H */

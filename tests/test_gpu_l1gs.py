@@ -1,0 +1,204 @@
+"""GPU parity of the L1GS build and sweeps (through the C ABI) against the oracle.
+
+Tolerance: FP64 results agree with the oracle within 1e-12 relative error;
+forward sweeps and every D_Dl1 / packed buffer are expected bit-identical
+(same summation order, no FMA contraction on either side), and are checked so.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+SWEEPS = ["forward", "backward", "symmetric"]
+
+
+def _hs_sweep(hs, name):
+    return {"forward": hs.ForwardSweep(), "backward": hs.BackwardSweep(),
+            "symmetric": hs.SymmetricSweep()}[name]
+
+
+def _random_sym(n, density, seed):
+    rng = np.random.default_rng(seed)
+    B = sp.random(n, n, density=density, random_state=rng, format="csc")
+    A = B + B.T
+    A = A + sp.diags(np.abs(A).sum(axis=1).A1 + 1.0 + rng.random(n))
+    A = A.tocsc()
+    A.sort_indices()
+    return A
+
+
+def _random_nonsym(n, density, seed):
+    rng = np.random.default_rng(seed)
+    B = sp.random(n, n, density=density, random_state=rng, format="csc")
+    B.data -= 0.5
+    A = B + sp.diags(np.abs(B).sum(axis=1).A1 * rng.uniform(0.3, 2.0, n) + 0.5)
+    A = A.tocsc()
+    A.sort_indices()
+    return A
+
+
+def test_known_answers_n9(gpu):
+    """test/test_preconditioners.jl:110-276 and test/gpu/gpu.jl:113-357."""
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.poisson_test_matrix(9)
+    x = np.arange(9, dtype=np.float64)
+    exp = {"forward": [0, 0.5, 1.0, 2.0, 2.0, 3.5, 3.0, 5.0, 4.0],
+           "backward": [0.25, 0.5, 1.75, 1.5, 3.25, 2.5, 4.75, 3.5, 4.0],
+           "symmetric": [0.25, 0.5, 2.0, 2.0, 3.75, 3.5, 5.5, 5.0, 4.0]}
+    for threads in (1, 3, 10):
+        for blocks in (1, 4, 10):
+            b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=threads, blocks=blocks)
+            for M in (A, A.tocsr()):
+                for sw in SWEEPS:
+                    for cache in (hs.PackedBufferCache(), hs.MatrixViewCache()):
+                        P = b(M, 2, isSymA=True, sweep=_hs_sweep(hs, sw), cache_strategy=cache)
+                        y = hs.backslash(P, x)
+                        np.testing.assert_array_equal(y, exp[sw])
+                        if sw == "symmetric":
+                            np.testing.assert_array_equal(P.sweep.lop.D_DL1, 2.0)
+                            np.testing.assert_array_equal(P.sweep.uop.D_DL1, 2.0)
+                            if isinstance(cache, hs.PackedBufferCache):
+                                np.testing.assert_array_equal(P.sweep.lop.L.SLbuffer, -1.0)
+                                np.testing.assert_array_equal(P.sweep.uop.U.SUbuffer, -1.0)
+                        else:
+                            np.testing.assert_array_equal(P.sweep.op.D_DL1, 2.0)
+
+
+def test_eta_nonsym_partsize_known_answers(gpu):
+    """test/test_preconditioners.jl:278-364."""
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.poisson_test_matrix(9)
+    x = np.arange(9, dtype=np.float64)
+    b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=2, blocks=2)
+    for eta, D in ((2.0, [2.0] * 9), (3.0, [2.0] + [2.5] * 8), (1.0, [2.0] * 9)):
+        P = b(A, 2, eta=eta, sweep=hs.ForwardSweep())
+        np.testing.assert_array_equal(P.sweep.op.D_DL1, D)
+        np.testing.assert_array_equal(P.sweep.op.L.SLbuffer, [-1.0] * 4)
+    A2 = A.tolil()
+    A2[0, 7] = -1.0
+    A2[1, 7] = -1.0
+    A2 = A2.tocsc()
+    fwd = [0, 1 / 3, 1.0, 2.0, 2.0, 3.5, 3.0, 5.0, 4.0]
+    bwd = [1 / 6, 1 / 3, 1.75, 1.5, 3.25, 2.5, 4.75, 3.5, 4.0]
+    for cache in (hs.PackedBufferCache(), hs.MatrixViewCache()):
+        P = b(A2, 2, sweep=hs.ForwardSweep(), cache_strategy=cache)
+        np.testing.assert_array_equal(P.sweep.op.D_DL1, [2, 3, 2, 2, 2, 2, 2, 2, 2])
+        np.testing.assert_allclose(hs.backslash(P, x), fwd, rtol=1e-15)
+        P = b(A2, 2, sweep=hs.BackwardSweep(), cache_strategy=cache)
+        np.testing.assert_allclose(hs.backslash(P, x), bwd, rtol=1e-15)
+    P = b(A, 3, sweep=hs.ForwardSweep())
+    np.testing.assert_array_equal(P.sweep.op.L.SLbuffer, [-1, 0, -1, -1, 0, -1, -1, 0, -1])
+
+
+@pytest.mark.parametrize("partsize", [1, 2, 3, 5, 10, 16, 17, 31, 32, 33, 63, 64, 65, 100])
+@pytest.mark.parametrize("kind", ["sym_csc", "csr", "nonsym_csc"])
+def test_random_parity(gpu, partsize, kind):
+    import oracle as O
+    hs = gpu
+    n = 517  # ragged last partition for most partsizes
+    if kind == "nonsym_csc":
+        A = _random_nonsym(n, 0.02, partsize)
+        is_sym = False
+    else:
+        A = _random_sym(n, 0.02, partsize)
+        is_sym = True
+    M = A.tocsr() if kind == "csr" else A
+    x = np.random.default_rng(7 + partsize).uniform(-1, 1, n)
+    b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=64, blocks=7)
+    for sw in SWEEPS:
+        if kind == "nonsym_csc" and sw == "symmetric":
+            pass  # allowed by the reference too
+        for cache in ("packed", "view"):
+            ref = O.l1gs_build(M, partsize, is_sym=is_sym, sweep=sw, cache=cache,
+                               gpu=(64, 7), fast_nonsym=False)
+            yo = O.l1gs_ldiv(ref, x)
+            P = b(M, partsize, isSymA=is_sym, sweep=_hs_sweep(hs, sw),
+                  cache_strategy=hs.PackedBufferCache() if cache == "packed" else hs.MatrixViewCache())
+            assert (P.partitioning.partsize, P.partitioning.nparts, P.partitioning.nchunks,
+                    P.partitioning.chunksize) == ref.part
+            y = hs.backslash(P, x)
+            D = P.sweep.lop.D_DL1 if sw == "symmetric" else P.sweep.op.D_DL1
+            np.testing.assert_array_equal(D, ref.D)
+            if sw == "forward":
+                np.testing.assert_array_equal(y, yo)
+            else:
+                assert relerr(y, yo) < TOL, (sw, cache, relerr(y, yo))
+            if cache == "packed":
+                if sw in ("forward", "symmetric"):
+                    L = P.sweep.op.L.SLbuffer if sw == "forward" else P.sweep.lop.L.SLbuffer
+                    np.testing.assert_array_equal(L, ref.SL)
+                if sw in ("backward", "symmetric"):
+                    U = P.sweep.op.U.SUbuffer if sw == "backward" else P.sweep.uop.U.SUbuffer
+                    np.testing.assert_array_equal(U, ref.SU)
+
+
+def test_device_vectors_and_inplace(gpu):
+    import torch
+    import oracle as O
+    hs = gpu
+    n = 1000
+    A = _random_sym(n, 0.01, 3)
+    x = np.random.default_rng(1).uniform(-1, 1, n)
+    ref = O.l1gs_build(A, 64, is_sym=True, sweep="symmetric", gpu=(256, 20))
+    yo = O.l1gs_ldiv(ref, x)
+    P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=20)(A, 64, isSymA=True)
+    xd = torch.tensor(x, device="cuda")
+    yd = torch.empty_like(xd)
+    hs.ldiv_(yd, P, xd)
+    assert relerr(yd.cpu().numpy(), yo) < TOL
+    np.testing.assert_array_equal(xd.cpu().numpy(), x)  # x unchanged
+    hs.ldiv_(xd, P, xd)  # aliasing y = x
+    np.testing.assert_array_equal(xd.cpu().numpy(), yd.cpu().numpy())
+
+
+def test_errors(gpu):
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=4, blocks=4)
+    A = pb.poisson_test_matrix(9)
+    with pytest.raises(hs.HSError, match="partsize must be greater than 0"):
+        b(A, 0)
+    A3 = A.copy()
+    A3.data[A3.indptr[4] + 1] = 0.0  # explicitly stored zero on the diagonal of column 5
+    assert A3.indices[A3.indptr[4] + 1] == 4
+    with pytest.raises(hs.SingularException) as ei:
+        b(A3, 2)  # non-symmetric path builds DiagonalIndices
+    assert ei.value.col == 5
+    A4 = A.tolil()
+    A4[6, 6] = 0.0
+    A4 = A4.tocsc()
+    A4.eliminate_zeros()
+    with pytest.raises(hs.SingularException) as ei:
+        b(A4, 2)  # missing diagonal entry
+    assert ei.value.col == 7
+
+
+def test_c2_poisson128_parity(gpu):
+    """3D 7-point Poisson 128^3, P=64, symmetric sweep, packed FP64 -- full size."""
+    import torch
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.poisson3d(128)
+    n = A.shape[0]
+    x = pb.uniform_rhs(n, 0)
+    ref = O.l1gs_build(A, 64, is_sym=True, sweep="symmetric", gpu=(256, 1024))
+    yo = O.l1gs_ldiv(ref, x)
+    P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=1024)(A, 64, isSymA=True)
+    xd = torch.tensor(x, device="cuda")
+    yd = torch.empty_like(xd)
+    hs.ldiv_(yd, P, xd)
+    y = yd.cpu().numpy()
+    # closed form of D_Dl1 on this stencil: 6, or 8.5 on the rows whose x-neighbour
+    # across the partition boundary counts twice (x in {63, 64}, interior y and z)
+    D = P.sweep.lop.D_DL1
+    np.testing.assert_array_equal(D, ref.D)
+    assert set(np.unique(D)) == {6.0, 8.5}
+    assert int((D == 8.5).sum()) == 2 * 126 * 126
+    np.testing.assert_array_equal(y, yo)  # one in-partition neighbour per side: exact
