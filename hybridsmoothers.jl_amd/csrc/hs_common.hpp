@@ -100,6 +100,10 @@ struct DBuf {
     T *get() const { return p; }
 };
 
+// Process-unique id of a handle (cached graphs are keyed by ids, never by
+// addresses that a later allocation could reuse).
+uint64_t next_serial();
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline void finish(hipStream_t s, int flags) {

@@ -9,6 +9,11 @@ namespace hs {
 std::atomic<int64_t> g_device_bytes{0};
 
 namespace {
+std::atomic<uint64_t> g_serial{0};
+}
+uint64_t next_serial() { return ++g_serial; }
+
+namespace {
 thread_local std::string t_err;
 thread_local int t_code = HS_OK;
 thread_local int64_t t_aux = 0;

@@ -21,7 +21,7 @@
 // of the reference's _accumulate_from_cache (:1547-1562) -- and the file is
 // compiled with -ffp-contract=off, so forward sweeps are bit-identical to the
 // reference arithmetic.  The backward recurrence adds in descending j.
-#include "hs_common.hpp"
+#include "hs_plans.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -149,12 +149,22 @@ __global__ void k_convert(int64_t n, const double *__restrict__ src, TD *__restr
 // ---------------------------------------------------------------------------
 // sweep kernels
 // ---------------------------------------------------------------------------
+// Broadcast of lane j of every PM-lane segment to the whole segment.
+// PM = 64: v_readlane into SGPRs; PM = 32: readlanes of lanes j and j+32 and a
+// per-half select; smaller segments: ds_bpermute (__shfl with width PM).
 template <int PM>
 __device__ __forceinline__ double seg_bcast(double v, int j) {
     if constexpr (PM == 64) {
         int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
         int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
         return __hiloint2double(hi, lo);
+    } else if constexpr (PM == 32) {
+        int lo0 = __builtin_amdgcn_readlane(__double2loint(v), j);
+        int hi0 = __builtin_amdgcn_readlane(__double2hiint(v), j);
+        int lo1 = __builtin_amdgcn_readlane(__double2loint(v), j + 32);
+        int hi1 = __builtin_amdgcn_readlane(__double2hiint(v), j + 32);
+        bool up = (threadIdx.x & 32) != 0;
+        return __hiloint2double(up ? hi1 : hi0, up ? lo1 : lo0);
     } else {
         return __shfl(v, j, PM);
     }
@@ -163,9 +173,202 @@ template <int PM>
 __device__ __forceinline__ float seg_bcast(float v, int j) {
     if constexpr (PM == 64) {
         return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+    } else if constexpr (PM == 32) {
+        int a = __builtin_amdgcn_readlane(__float_as_int(v), j);
+        int b = __builtin_amdgcn_readlane(__float_as_int(v), j + 32);
+        return __int_as_float((threadIdx.x & 32) ? b : a);
     } else {
         return __shfl(v, j, PM);
     }
+}
+
+// v_writelane_b32 (no clang builtin exists; this binds the LLVM intrinsic).
+extern "C" __device__ int hs_writelane_i32(int value, int lane, int old)
+    __asm("llvm.amdgcn.writelane.i32");
+
+// Broadcast lane j of every segment and record the broadcast value in lane j
+// of `keep` (v_writelane from the SGPR copy: no per-step lane masks).
+template <int PM>
+__device__ __forceinline__ double bcast_keep(double v, int j, double &keep) {
+    if constexpr (PM == 64) {
+        int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+        int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+        int klo = hs_writelane_i32(lo, j, __double2loint(keep));
+        int khi = hs_writelane_i32(hi, j, __double2hiint(keep));
+        keep = __hiloint2double(khi, klo);
+        return __hiloint2double(hi, lo);
+    } else if constexpr (PM == 32) {
+        int lo0 = __builtin_amdgcn_readlane(__double2loint(v), j);
+        int hi0 = __builtin_amdgcn_readlane(__double2hiint(v), j);
+        int lo1 = __builtin_amdgcn_readlane(__double2loint(v), j + 32);
+        int hi1 = __builtin_amdgcn_readlane(__double2hiint(v), j + 32);
+        int klo = hs_writelane_i32(lo0, j, __double2loint(keep));
+        int khi = hs_writelane_i32(hi0, j, __double2hiint(keep));
+        klo = hs_writelane_i32(lo1, j + 32, klo);
+        khi = hs_writelane_i32(hi1, j + 32, khi);
+        keep = __hiloint2double(khi, klo);
+        bool up = (threadIdx.x & 32) != 0;
+        return __hiloint2double(up ? hi1 : hi0, up ? lo1 : lo0);
+    } else {
+        const int lane = threadIdx.x % PM;
+        keep = (lane == j) ? v : keep;
+        return __shfl(v, j, PM);
+    }
+}
+template <int PM>
+__device__ __forceinline__ float bcast_keep(float v, int j, float &keep) {
+    if constexpr (PM == 64) {
+        int b = __builtin_amdgcn_readlane(__float_as_int(v), j);
+        keep = __int_as_float(hs_writelane_i32(b, j, __float_as_int(keep)));
+        return __int_as_float(b);
+    } else if constexpr (PM == 32) {
+        int a = __builtin_amdgcn_readlane(__float_as_int(v), j);
+        int b = __builtin_amdgcn_readlane(__float_as_int(v), j + 32);
+        int k = hs_writelane_i32(a, j, __float_as_int(keep));
+        keep = __int_as_float(hs_writelane_i32(b, j + 32, k));
+        return __int_as_float((threadIdx.x & 32) ? b : a);
+    } else {
+        const int lane = threadIdx.x % PM;
+        keep = (lane == j) ? v : keep;
+        return __shfl(v, j, PM);
+    }
+}
+
+// Correctly rounded z/d from the correctly rounded reciprocal r = 1/d:
+// q0 = z*r, e = z - q0*d (exact with an FMA), q = q0 + e*r rounds to z/d
+// (Markstein's theorem) whenever d, r and z/d are normal numbers -- the
+// sweep checks |d| once per partition and otherwise uses true division.
+__device__ __forceinline__ double quot_fast(double z, double d, double r) {
+    double q0 = z * r;
+    double e = __builtin_fma(-q0, d, z);
+    return __builtin_fma(e, r, q0);
+}
+__device__ __forceinline__ float quot_fast(float z, float d, float r) {
+    float q0 = z * r;
+    float e = __builtin_fmaf(-q0, d, z);
+    return __builtin_fmaf(e, r, q0);
+}
+template <bool FAST, class TC>
+__device__ __forceinline__ TC quot(TC z, TC d, TC r) {
+    if constexpr (FAST) return quot_fast(z, d, r);
+    else return z / d;
+}
+__device__ __forceinline__ bool safe_divisor(double d) {
+    double a = fabs(d);
+    return a >= 0x1p-400 && a <= 0x1p400;
+}
+__device__ __forceinline__ bool safe_divisor(float d) {
+    float a = fabsf(d);
+    return a >= 0x1p-60f && a <= 0x1p60f;
+}
+
+// Any partition shape (last, short partition; P below the segment width):
+// steps guarded by the actual size Pa.
+template <int PM, int MODE, bool FAST, class TS, class TC>
+__device__ __forceinline__ TC sweep_generic(TC xr, TC dr, TC rr, int lane, int Pa, bool live,
+                                            const TS *__restrict__ Lk,
+                                            const TS *__restrict__ Uk) {
+    TC yr = xr;
+    if constexpr (MODE != MODE_BWD) {
+        TC acc = (TC)0, yfin = (TC)0;
+#pragma unroll
+        for (int j = 0; j < PM; ++j) {
+            if (j < Pa) {
+                TC q = quot<FAST>(xr - acc, dr, rr);
+                yfin = (lane == j) ? q : yfin;
+                if (j < Pa - 1) {
+                    TC yj = seg_bcast<PM>(q, j);
+                    bool has = (lane > j) && live;
+                    TC l = has ? load_as<TC>(Lk[colL_off(j, Pa) + (lane - j - 1)]) : (TC)0;
+                    if (lane > j) acc = acc + l * yj;
+                }
+            }
+        }
+        yr = yfin;
+    }
+    if constexpr (MODE == MODE_SYM) xr = dr * yr;  // y .= D_Dl1 .* y  (:897)
+    if constexpr (MODE != MODE_FWD) {
+        TC acc = (TC)0, yfin = (TC)0;
+#pragma unroll
+        for (int jj = 0; jj < PM; ++jj) {
+            const int j = PM - 1 - jj;
+            if (j < Pa) {
+                TC q = quot<FAST>(xr - acc, dr, rr);
+                yfin = (lane == j) ? q : yfin;
+                if (j > 0) {
+                    TC yj = seg_bcast<PM>(q, j);
+                    TC u = (lane < j) ? load_as<TC>(Uk[colU_off(j) + lane]) : (TC)0;
+                    if (lane < j) acc = acc + u * yj;
+                }
+            }
+        }
+        yr = yfin;
+    }
+    return yr;
+}
+
+// Full partitions (Pa == PM): straight-line, branch-free recurrence.  The
+// column loads of both passes form one stream that runs RING columns ahead
+// of the step consuming them, so every wave keeps RING*64 values in flight
+// while it computes and the compiler can wait on exact load counts.
+template <int PM, int MODE, int RING, class TS, class TC>
+__device__ __forceinline__ TC sweep_full(TC xr, TC dr, TC rr, int lane,
+                                         const TS *__restrict__ Lk, const TS *__restrict__ Uk) {
+    constexpr int NL = (MODE != MODE_BWD) ? PM - 1 : 0;  // forward columns 0..PM-2
+    constexpr int NU = (MODE != MODE_FWD) ? PM - 1 : 0;  // backward columns PM-1..1
+    constexpr int NC = NL + NU;
+    constexpr int R = RING < NC ? RING : NC;
+    // Unconditional loads: lanes whose row is already final (i <= j forward,
+    // i >= j backward) read a neighbouring in-block value into an accumulator
+    // that is never used again.  Indices stay inside block k: forward column j
+    // reads colL(j) + lane - j - 1 >= 0 for j >= 1 (clamped for j = 0);
+    // backward column j reads colU(j) + lane <= P(P-1)/2 - 1 for j <= P-2
+    // (clamped for j = P-1).
+    auto load_col = [&](int c) -> TS {
+        if (c < NL) {
+            const int j = c;
+            const int off = (j == 0) ? (lane > 0 ? lane - 1 : 0) : lane - j - 1;
+            return Lk[colL_off(j, PM) + off];
+        }
+        const int j = PM - 1 - (c - NL);
+        const int off = (j == PM - 1) ? (lane < PM - 2 ? lane : PM - 2) : lane;
+        return Uk[colU_off(j) + off];
+    };
+    TS ring[R > 0 ? R : 1];
+#pragma unroll
+    for (int c = 0; c < R; ++c) ring[c] = load_col(c);
+
+    TC yr = xr;
+    if constexpr (MODE != MODE_BWD) {
+        TC acc = (TC)0, yfin = (TC)0;
+#pragma unroll
+        for (int j = 0; j < PM - 1; ++j) {
+            const TS v = ring[j % R];
+            if (j + R < NC) ring[j % R] = load_col(j + R);
+            TC q = quot_fast(xr - acc, dr, rr);
+            TC yj = bcast_keep<PM>(q, j, yfin);
+            acc = acc + load_as<TC>(v) * yj;  // dead for lanes <= j
+        }
+        TC q = quot_fast(xr - acc, dr, rr);
+        yr = (lane == PM - 1) ? q : yfin;
+    }
+    if constexpr (MODE == MODE_SYM) xr = dr * yr;  // y .= D_Dl1 .* y  (:897)
+    if constexpr (MODE != MODE_FWD) {
+        TC acc = (TC)0, yfin = (TC)0;
+#pragma unroll
+        for (int jj = 0; jj < PM - 1; ++jj) {
+            const int j = PM - 1 - jj;
+            const int c = NL + jj;
+            const TS v = ring[c % R];
+            if (c + R < NC) ring[c % R] = load_col(c + R);
+            TC q = quot_fast(xr - acc, dr, rr);
+            TC yj = bcast_keep<PM>(q, j, yfin);
+            acc = acc + load_as<TC>(v) * yj;  // dead for lanes >= j
+        }
+        TC q = quot_fast(xr - acc, dr, rr);
+        yr = (lane == 0) ? q : yfin;
+    }
+    return yr;
 }
 
 // One partition per PM-lane segment, 256 threads per workgroup.
@@ -176,7 +379,8 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
                                                    const TS *__restrict__ Lc,
                                                    const TS *__restrict__ Uc,
                                                    const TC *__restrict__ D, int64_t n, int P,
-                                                   int64_t nparts) {
+                                                   int64_t nparts, const int *__restrict__ done) {
+    if (done && *done) return;  // converged CG: speculative launches do nothing
     constexpr int SEGS = 256 / PM;
     const int lane = threadIdx.x % PM;
     const int64_t k = (int64_t)blockIdx.x * SEGS + threadIdx.x / PM;
@@ -186,52 +390,19 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
     const int Pa = (int)(rem < P ? rem : P);
     const int64_t boff = k * (((int64_t)P * (P - 1)) / 2);
     const bool live = lane < Pa;
+    const TS *Lk = Lc + boff;
+    const TS *Uk = Uc + boff;
 
-    TC xr = live ? (TC)x[s + lane] : (TC)0;
-    TC dr = live ? D[s + lane] : (TC)1;
-    TC yr = xr;
-
-    if constexpr (MODE != MODE_BWD) {
-        const TS *Lk = Lc + boff;
-        TC l[PM];
-#pragma unroll
-        for (int j = 0; j < PM; ++j) {
-            bool has = (lane > j) && live;
-            l[j] = has ? load_as<TC>(Lk[colL_off(j, Pa) + (lane - j - 1)]) : (TC)0;
-        }
-        TC acc = (TC)0;
-#pragma unroll
-        for (int j = 0; j < PM - 1; ++j) {
-            if (j < Pa - 1) {
-                TC yv = (xr - acc) / dr;
-                TC yj = seg_bcast<PM>(yv, j);
-                if (lane > j) acc = acc + l[j] * yj;
-            }
-        }
-        yr = (xr - acc) / dr;
-    }
-    if constexpr (MODE == MODE_SYM) {
-        xr = dr * yr;  // y .= D_Dl1 .* y  (:897)
-    }
-    if constexpr (MODE != MODE_FWD) {
-        const TS *Uk = Uc + boff;
-        TC u[PM];
-#pragma unroll
-        for (int j = 0; j < PM; ++j) {
-            bool has = (lane < j) && (j < Pa);
-            u[j] = has ? load_as<TC>(Uk[colU_off(j) + lane]) : (TC)0;
-        }
-        TC acc = (TC)0;
-#pragma unroll
-        for (int jj = 0; jj < PM - 1; ++jj) {
-            const int j = PM - 1 - jj;
-            if (j < Pa) {
-                TC yv = (xr - acc) / dr;
-                TC yj = seg_bcast<PM>(yv, j);
-                if (lane < j) acc = acc + u[j] * yj;
-            }
-        }
-        yr = (xr - acc) / dr;
+    const TC xr = live ? (TC)x[s + lane] : (TC)0;
+    const TC dr = live ? D[s + lane] : (TC)1;
+    const TC rr = (TC)1 / dr;
+    TC yr;
+    if (__any(!safe_divisor(dr))) {
+        yr = sweep_generic<PM, MODE, false, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
+    } else if (Pa == PM) {
+        yr = sweep_full<PM, MODE, 16, TS, TC>(xr, dr, rr, lane, Lk, Uk);
+    } else {
+        yr = sweep_generic<PM, MODE, true, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
     }
     if (live) y[s + lane] = (double)yr;
 }
@@ -242,7 +413,8 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
 template <int MODE, class TS, class TC>
 __global__ void k_sweep_thread(const double *x, double *y, const TS *__restrict__ Lc,
                                const TS *__restrict__ Uc, const TC *__restrict__ D, int64_t n,
-                               int64_t P, int64_t nparts) {
+                               int64_t P, int64_t nparts, const int *__restrict__ done) {
+    if (done && *done) return;
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nparts) return;
     int64_t s = k * P;
@@ -321,7 +493,9 @@ template <int MODE>
 __global__ void k_sweep_view(const double *x, double *y, const int64_t *__restrict__ ptr,
                              const int64_t *__restrict__ ind, const double *__restrict__ val,
                              const int64_t *__restrict__ diag, int rowwise,
-                             const double *__restrict__ D, int64_t n, int64_t P, int64_t nparts) {
+                             const double *__restrict__ D, int64_t n, int64_t P, int64_t nparts,
+                             const int *__restrict__ done) {
+    if (done && *done) return;
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nparts) return;
     int64_t s = k * P;
@@ -342,6 +516,7 @@ __global__ void k_sweep_view(const double *x, double *y, const int64_t *__restri
 // plan
 // ---------------------------------------------------------------------------
 struct L1GSPlan {
+    uint64_t serial = next_serial();
     int64_t n = 0, P = 1, nparts = 0, nchunks = 1, chunksize = 1, S = 0;
     int sweep = HS_SWEEP_SYMMETRIC, cache = HS_CACHE_PACKED, prec = HS_PREC_FP64;
     int fmt = HS_FMT_CSC, sym = 0, rowwise = 1;
@@ -357,7 +532,8 @@ struct L1GSPlan {
 };
 
 template <int PM, class TS, class TC>
-static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *D, hipStream_t st) {
+static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *D, hipStream_t st,
+                       const int *done) {
     constexpr int SEGS = 256 / PM;
     unsigned grid = grid_for(h.nparts, SEGS);
     const TS *L = reinterpret_cast<const TS *>(h.L.get());
@@ -366,45 +542,46 @@ static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *
     switch (h.sweep) {
     case HS_SWEEP_FORWARD:
         hipLaunchKernelGGL((k_sweep_seg<PM, MODE_FWD, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
-                           L, U, D, h.n, P, h.nparts);
+                           L, U, D, h.n, P, h.nparts, done);
         break;
     case HS_SWEEP_BACKWARD:
         hipLaunchKernelGGL((k_sweep_seg<PM, MODE_BWD, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
-                           L, U, D, h.n, P, h.nparts);
+                           L, U, D, h.n, P, h.nparts, done);
         break;
     default:
         hipLaunchKernelGGL((k_sweep_seg<PM, MODE_SYM, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
-                           L, U, D, h.n, P, h.nparts);
+                           L, U, D, h.n, P, h.nparts, done);
         break;
     }
 }
 
 template <class TS, class TC>
 static void launch_packed(const L1GSPlan &h, const double *x, double *y, const TC *D,
-                          hipStream_t st) {
-    if (h.P <= 4) launch_seg<4, TS, TC>(h, x, y, D, st);
-    else if (h.P <= 8) launch_seg<8, TS, TC>(h, x, y, D, st);
-    else if (h.P <= 16) launch_seg<16, TS, TC>(h, x, y, D, st);
-    else if (h.P <= 32) launch_seg<32, TS, TC>(h, x, y, D, st);
-    else if (h.P <= 64) launch_seg<64, TS, TC>(h, x, y, D, st);
+                          hipStream_t st, const int *done) {
+    if (h.P <= 4) launch_seg<4, TS, TC>(h, x, y, D, st, done);
+    else if (h.P <= 8) launch_seg<8, TS, TC>(h, x, y, D, st, done);
+    else if (h.P <= 16) launch_seg<16, TS, TC>(h, x, y, D, st, done);
+    else if (h.P <= 32) launch_seg<32, TS, TC>(h, x, y, D, st, done);
+    else if (h.P <= 64) launch_seg<64, TS, TC>(h, x, y, D, st, done);
     else {
         const TS *L = reinterpret_cast<const TS *>(h.L.get());
         const TS *U = reinterpret_cast<const TS *>(h.U.get());
         unsigned grid = grid_for(h.nparts, 64);
         if (h.sweep == HS_SWEEP_FORWARD)
             hipLaunchKernelGGL((k_sweep_thread<MODE_FWD, TS, TC>), dim3(grid), dim3(64), 0, st, x,
-                               y, L, U, D, h.n, h.P, h.nparts);
+                               y, L, U, D, h.n, h.P, h.nparts, done);
         else if (h.sweep == HS_SWEEP_BACKWARD)
             hipLaunchKernelGGL((k_sweep_thread<MODE_BWD, TS, TC>), dim3(grid), dim3(64), 0, st, x,
-                               y, L, U, D, h.n, h.P, h.nparts);
+                               y, L, U, D, h.n, h.P, h.nparts, done);
         else
             hipLaunchKernelGGL((k_sweep_thread<MODE_SYM, TS, TC>), dim3(grid), dim3(64), 0, st, x,
-                               y, L, U, D, h.n, h.P, h.nparts);
+                               y, L, U, D, h.n, h.P, h.nparts, done);
     }
 }
 
 // Device-side application y = M^-1 x (x, y device pointers; may alias).
-void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_t st) {
+void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_t st,
+                       const int *done) {
     if (h.n == 0) return;
     if (h.cache == HS_CACHE_VIEW) {
         unsigned grid = grid_for(h.nparts, 64);
@@ -412,21 +589,21 @@ void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_
         if (h.sweep == HS_SWEEP_FORWARD)
             hipLaunchKernelGGL(k_sweep_view<MODE_FWD>, dim3(grid), dim3(64), 0, st, x, y,
                                h.vptr.get(), h.vind.get(), h.vval.get(), dg, h.rowwise, h.D.get(),
-                               h.n, h.P, h.nparts);
+                               h.n, h.P, h.nparts, done);
         else if (h.sweep == HS_SWEEP_BACKWARD)
             hipLaunchKernelGGL(k_sweep_view<MODE_BWD>, dim3(grid), dim3(64), 0, st, x, y,
                                h.vptr.get(), h.vind.get(), h.vval.get(), dg, h.rowwise, h.D.get(),
-                               h.n, h.P, h.nparts);
+                               h.n, h.P, h.nparts, done);
         else
             hipLaunchKernelGGL(k_sweep_view<MODE_SYM>, dim3(grid), dim3(64), 0, st, x, y,
                                h.vptr.get(), h.vind.get(), h.vval.get(), dg, h.rowwise, h.D.get(),
-                               h.n, h.P, h.nparts);
+                               h.n, h.P, h.nparts, done);
     } else if (h.prec == HS_PREC_FP64) {
-        launch_packed<double, double>(h, x, y, h.D.get(), st);
+        launch_packed<double, double>(h, x, y, h.D.get(), st, done);
     } else if (h.prec == HS_PREC_FP32) {
-        launch_packed<float, float>(h, x, y, h.Df.get(), st);
+        launch_packed<float, float>(h, x, y, h.Df.get(), st, done);
     } else {
-        launch_packed<__hip_bfloat16, float>(h, x, y, h.Df.get(), st);
+        launch_packed<__hip_bfloat16, float>(h, x, y, h.Df.get(), st, done);
     }
     check_launch();
 }
@@ -467,6 +644,9 @@ static void nonsym_D(L1GSPlan &h, const int64_t *h_ptr, const int64_t *h_ind, co
     check_launch();
     HS_CHECK_HIP(hipStreamSynchronize(st));  // the host arrays above go out of scope
 }
+
+int64_t l1gs_rows(const L1GSPlan &h) { return h.n; }
+uint64_t l1gs_serial(const L1GSPlan &h) { return h.serial; }
 
 template <class TS>
 static void build_packed(L1GSPlan &h, const DBuf<int64_t> &ptr, const DBuf<int64_t> &ind,
@@ -618,14 +798,14 @@ int hs_l1gs_apply(hs_l1gs_t h, double *y, const double *x, int loc, void *stream
         hipStream_t st = as_stream(stream);
         if (h->n == 0) return;
         if (loc == HS_LOC_DEVICE) {
-            l1gs_apply_device(*h, x, y, st);
+            l1gs_apply_device(*h, x, y, st, nullptr);
             finish(st, flags);
         } else {
             // host vectors: y .= x; adapt; sweep; copyto!(y, _y)  (:367-384)
             size_t bytes = (size_t)h->n * 8;
             h->stage.ensure(h->n);
             HS_CHECK_HIP(hipMemcpyAsync(h->stage.a.get(), x, bytes, hipMemcpyHostToDevice, st));
-            l1gs_apply_device(*h, h->stage.a.get(), h->stage.a.get(), st);
+            l1gs_apply_device(*h, h->stage.a.get(), h->stage.a.get(), st, nullptr);
             HS_CHECK_HIP(hipMemcpyAsync(y, h->stage.a.get(), bytes, hipMemcpyDeviceToHost, st));
             HS_CHECK_HIP(hipStreamSynchronize(st));
         }

@@ -1,0 +1,401 @@
+// hs_pcg.hip -- FP64 preconditioned CG on the device, the solver the
+// reference's users reach through solve(LinearProblem(A, b), KrylovJL_CG(); Pl = P)
+// (README.md:27-29).  The iteration is the one of Krylov.jl `cg` with
+// ldiv = true and x0 = 0:
+//
+//   r = b; z = M^-1 r; p = z; gamma = r'z; eps = atol + rtol*sqrt(gamma)
+//   loop: q = A p; alpha = gamma / p'q; x += alpha p; r -= alpha q;
+//         z = M^-1 r; gamma' = r'z; stop if sqrt(gamma') <= eps;
+//         beta = gamma'/gamma; p = z + beta p
+//
+// All scalars live on the device; every kernel first reads the `done` flag,
+// so blocks of iterations are launched (and captured in a HIP graph) without
+// a host round trip per iteration, and launches past convergence do nothing.
+// Dot products are deterministic (fixed block tree, fixed final order).
+#include "hs_plans.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <memory>
+
+namespace hs {
+
+struct CGDev {
+    double gamma, delta, gnext, beta;
+    double rnorm, rnorm0, eps, atol, rtol;
+    double res2, b2;
+    long long iter, itmax;
+    int done, status;
+    unsigned cnt[4];
+};
+
+__global__ void k_cg_start(int64_t n, const double *__restrict__ b, double *__restrict__ x,
+                           double *__restrict__ r) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        x[i] = 0.0;
+        r[i] = b[i];
+    }
+}
+
+// p = z; gamma = r'z; rnorm0; eps
+__global__ __launch_bounds__(256) void k_cg_dot0(int64_t n, const double *__restrict__ r,
+                                                 const double *__restrict__ z,
+                                                 double *__restrict__ p, double *partials,
+                                                 CGDev *st) {
+    __shared__ double sm[4];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        double zi = z[i];
+        p[i] = zi;
+        v = r[i] * zi;
+    }
+    double bs = block_sum(v, sm);
+    double tot;
+    if (grid_sum(bs, partials, &st->cnt[0], &st->gamma, sm, &tot) && threadIdx.x == 0) {
+        double rn = sqrt(tot);
+        st->rnorm0 = rn;
+        st->rnorm = rn;
+        st->eps = st->atol + st->rtol * rn;
+        st->iter = 0;
+        st->status = 0;
+        st->done = (tot == 0.0 || rn <= st->eps) ? 1 : 0;
+        if (!st->done && st->itmax <= 0) {
+            st->done = 1;
+            st->status = 1;
+        }
+    }
+}
+
+// q = A p; delta = p'q
+__global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, const int64_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ ci,
+                                                 const double *__restrict__ v,
+                                                 const double *__restrict__ p,
+                                                 double *__restrict__ q, double *partials,
+                                                 CGDev *st) {
+    __shared__ double sm[4];
+    if (st->done) return;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.0;
+    if (i < n) {
+        double qi = row_dot(rp, ci, v, p, i);
+        q[i] = qi;
+        d = p[i] * qi;
+    }
+    double bs = block_sum(d, sm);
+    double tot;
+    if (grid_sum(bs, partials, &st->cnt[1], &st->delta, sm, &tot) && threadIdx.x == 0) {
+        if (!(tot > 0.0)) {  // breakdown: A (or M) not positive definite
+            st->done = 1;
+            st->status = 2;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha q
+__global__ void k_cg_axpy(int64_t n, const double *__restrict__ p, const double *__restrict__ q,
+                          double *__restrict__ x, double *__restrict__ r, const CGDev *st) {
+    if (st->done) return;
+    const double alpha = st->gamma / st->delta;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        x[i] = x[i] + alpha * p[i];
+        r[i] = r[i] - alpha * q[i];
+    }
+}
+
+// gamma' = r'z, stopping test, beta
+__global__ __launch_bounds__(256) void k_cg_dot(int64_t n, const double *__restrict__ r,
+                                                const double *__restrict__ z, double *partials,
+                                                CGDev *st) {
+    __shared__ double sm[4];
+    if (st->done) return;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = (i < n) ? r[i] * z[i] : 0.0;
+    double bs = block_sum(v, sm);
+    double tot;
+    if (grid_sum(bs, partials, &st->cnt[2], &st->gnext, sm, &tot) && threadIdx.x == 0) {
+        double rn = sqrt(tot);
+        st->rnorm = rn;
+        st->iter += 1;
+        bool solved = (rn <= st->eps) || (rn + 1.0 <= 1.0);
+        if (solved) {
+            st->done = 1;
+            st->status = 0;
+        } else {
+            st->beta = tot / st->gamma;
+            st->gamma = tot;
+            if (st->iter >= st->itmax) {
+                st->done = 1;
+                st->status = 1;
+            }
+        }
+    }
+}
+
+// p = z + beta p
+__global__ void k_cg_pupd(int64_t n, const double *__restrict__ z, double *__restrict__ p,
+                          const CGDev *st) {
+    if (st->done) return;
+    const double beta = st->beta;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+__global__ void k_copy(int64_t n, const double *__restrict__ a, double *__restrict__ b,
+                       const CGDev *st) {
+    if (st && st->done) return;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// ||b - A x||^2 and ||b||^2
+__global__ __launch_bounds__(256) void k_cg_resid(int64_t n, const int64_t *__restrict__ rp,
+                                                  const int32_t *__restrict__ ci,
+                                                  const double *__restrict__ v,
+                                                  const double *__restrict__ x,
+                                                  const double *__restrict__ b,
+                                                  double *partials, CGDev *st) {
+    __shared__ double sm[4];
+    __shared__ unsigned s_ticket;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double t2 = 0.0, b2 = 0.0;
+    if (i < n) {
+        double bi = b[i];
+        double t = bi - row_dot(rp, ci, v, x, i);
+        t2 = t * t;
+        b2 = bi * bi;
+    }
+    double s1 = block_sum(t2, sm);
+    double s2 = block_sum(b2, sm);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = s1;
+        partials[2 * blockIdx.x + 1] = s2;
+        __threadfence();
+        s_ticket = atomicAdd(&st->cnt[3], 1u);
+    }
+    __syncthreads();
+    if (s_ticket != gridDim.x - 1) return;
+    __threadfence();
+    double a = 0.0, c = 0.0;
+    for (unsigned blk = threadIdx.x; blk < gridDim.x; blk += blockDim.x) {
+        a += partials[2 * blk];
+        c += partials[2 * blk + 1];
+    }
+    a = block_sum(a, sm);
+    c = block_sum(c, sm);
+    if (threadIdx.x == 0) {
+        st->res2 = a;
+        st->b2 = c;
+        st->cnt[3] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// workspace: vectors, scalars, a private stream for graph capture
+// ---------------------------------------------------------------------------
+struct CGWork {
+    int64_t n = 0;
+    DBuf<double> x, r, z, p, q, b, partials;
+    DBuf<CGDev> st;
+    CGDev *host = nullptr;  // pinned mirror of the scalars
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t graph = nullptr;
+    uint64_t graph_key[3] = {0, 0, 0};  // matrix serial, preconditioner serial, kind
+    int graph_block = 0;
+
+    void ensure(int64_t nn) {
+        if (!stream) {
+            HS_CHECK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            HS_CHECK_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+            HS_CHECK_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+            HS_CHECK_HIP(hipHostMalloc((void **)&host, sizeof(CGDev), hipHostMallocDefault));
+            st.alloc(1);
+        }
+        if (nn != n) {
+            n = nn;
+            x.alloc(n); r.alloc(n); z.alloc(n); p.alloc(n); q.alloc(n); b.alloc(n);
+            partials.alloc(2 * (size_t)grid_for(n, 256));
+            drop_graph();
+        }
+    }
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_key[0] = graph_key[1] = graph_key[2] = 0;
+    }
+    ~CGWork() {
+        drop_graph();
+        if (host) (void)hipHostFree(host);
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+struct PrecRef {
+    int kind;
+    void *plan;
+    uint64_t serial() const {
+        if (kind == HS_PCG_L1GS) return l1gs_serial(*reinterpret_cast<L1GSPlan *>(plan));
+        if (kind == HS_PCG_CHEB) return cheb_serial(*reinterpret_cast<ChebPlan *>(plan));
+        return 0;
+    }
+};
+
+static void apply_prec(const PrecRef &M, const double *r, double *z, int64_t n, hipStream_t st,
+                       const int *done, const CGDev *dst) {
+    switch (M.kind) {
+    case HS_PCG_L1GS:
+        l1gs_apply_device(*reinterpret_cast<L1GSPlan *>(M.plan), r, z, st, done);
+        break;
+    case HS_PCG_CHEB:
+        cheb_apply_device(*reinterpret_cast<ChebPlan *>(M.plan), z, r, true, st, done);
+        break;
+    default:
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256)), dim3(256), 0, st, n, r, z, dst);
+        check_launch();
+    }
+}
+
+static void enqueue_iteration(const CSR &A, const PrecRef &M, CGWork &w, hipStream_t st) {
+    const int64_t n = w.n;
+    const unsigned g = grid_for(n, 256);
+    CGDev *d = w.st.get();
+    const int *done = &d->done;
+    hipLaunchKernelGGL(k_cg_spmv, dim3(g), dim3(256), 0, st, n, A.rowptr.get(), A.col.get(),
+                       A.val.get(), w.p.get(), w.q.get(), w.partials.get(), d);
+    hipLaunchKernelGGL(k_cg_axpy, dim3(g), dim3(256), 0, st, n, w.p.get(), w.q.get(), w.x.get(),
+                       w.r.get(), d);
+    check_launch();
+    apply_prec(M, w.r.get(), w.z.get(), n, st, done, d);
+    hipLaunchKernelGGL(k_cg_dot, dim3(g), dim3(256), 0, st, n, w.r.get(), w.z.get(),
+                       w.partials.get(), d);
+    hipLaunchKernelGGL(k_cg_pupd, dim3(g), dim3(256), 0, st, n, w.z.get(), w.p.get(), d);
+    check_launch();
+}
+
+// Solves with x0 = 0; bvec/xvec are device pointers.  Iterations run in
+// blocks of `block` launches between host checks of the done flag; with
+// use_graph the block is captured once into a HIP graph and replayed.
+void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, double *xvec,
+                double rtol, double atol, int64_t itmax, int block, bool use_graph,
+                hipStream_t user, hs_pcg_stats *stats) {
+    const int64_t n = A.n;
+    w.ensure(n);
+    hipStream_t st = w.stream;
+    const unsigned g = grid_for(n, 256);
+    auto t0 = std::chrono::steady_clock::now();
+    HS_CHECK_HIP(hipEventRecord(w.ev_in, user));
+    HS_CHECK_HIP(hipStreamWaitEvent(st, w.ev_in, 0));
+
+    CGDev init{};
+    init.atol = atol;
+    init.rtol = rtol;
+    init.itmax = itmax;
+    *w.host = init;
+    HS_CHECK_HIP(hipMemcpyAsync(w.st.get(), w.host, sizeof(CGDev), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_cg_start, dim3(g), dim3(256), 0, st, n, bvec, w.x.get(), w.r.get());
+    check_launch();
+    apply_prec(M, w.r.get(), w.z.get(), n, st, nullptr, nullptr);
+    hipLaunchKernelGGL(k_cg_dot0, dim3(g), dim3(256), 0, st, n, w.r.get(), w.z.get(), w.p.get(),
+                       w.partials.get(), w.st.get());
+    check_launch();
+
+    if (block < 1) block = 1;
+    if (use_graph) {
+        const uint64_t key[3] = {A.serial, M.serial(), (uint64_t)M.kind};
+        if (!w.graph || w.graph_block != block || key[0] != w.graph_key[0] ||
+            key[1] != w.graph_key[1] || key[2] != w.graph_key[2]) {
+            w.drop_graph();
+            hipGraph_t graph;
+            HS_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < block; ++k) enqueue_iteration(A, M, w, st);
+            HS_CHECK_HIP(hipStreamEndCapture(st, &graph));
+            HS_CHECK_HIP(hipGraphInstantiate(&w.graph, graph, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(graph);
+            w.graph_block = block;
+            for (int k = 0; k < 3; ++k) w.graph_key[k] = key[k];
+        }
+    }
+    for (;;) {
+        HS_CHECK_HIP(hipMemcpyAsync(w.host, w.st.get(), sizeof(CGDev), hipMemcpyDeviceToHost, st));
+        HS_CHECK_HIP(hipStreamSynchronize(st));
+        if (w.host->done) break;
+        if (use_graph) {
+            HS_CHECK_HIP(hipGraphLaunch(w.graph, st));
+        } else {
+            for (int k = 0; k < block; ++k) enqueue_iteration(A, M, w, st);
+        }
+    }
+    // true residual ||b - A x|| / ||b|| in FP64
+    hipLaunchKernelGGL(k_cg_resid, dim3(g), dim3(256), 0, st, n, A.rowptr.get(), A.col.get(),
+                       A.val.get(), w.x.get(), bvec, w.partials.get(), w.st.get());
+    check_launch();
+    HS_CHECK_HIP(hipMemcpyAsync(xvec, w.x.get(), n * 8, hipMemcpyDeviceToDevice, st));
+    HS_CHECK_HIP(hipMemcpyAsync(w.host, w.st.get(), sizeof(CGDev), hipMemcpyDeviceToHost, st));
+    HS_CHECK_HIP(hipStreamSynchronize(st));
+    HS_CHECK_HIP(hipEventRecord(w.ev_out, st));
+    HS_CHECK_HIP(hipStreamWaitEvent(user, w.ev_out, 0));
+    auto t1 = std::chrono::steady_clock::now();
+    const CGDev &h = *w.host;
+    stats->iters = h.iter;
+    stats->status = h.status;
+    stats->rnorm0 = h.rnorm0;
+    stats->rnorm = h.rnorm;
+    stats->true_relres = h.b2 > 0 ? std::sqrt(h.res2 / h.b2) : std::sqrt(h.res2);
+    stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+void cgwork_free(CGWork *w) { delete w; }
+
+// The CG workspace of a matrix handle: created on first use, freed with the
+// handle.  Concurrent solves with one matrix handle are not supported.
+static CGWork &work_for(CSR &A) {
+    if (!A.cg) A.cg = new CGWork();
+    return *A.cg;
+}
+
+}  // namespace hs
+
+using namespace hs;
+
+extern "C" {
+
+int hs_pcg_solve(hs_csr_t A, int prec_kind, void *prec, const double *b, double *x, double rtol,
+                 double atol, int64_t itmax, int loc, void *stream, hs_pcg_stats *stats) {
+    return guard([&] {
+        HS_REQUIRE(A && b && x && stats, HS_EINVAL, "NULL argument");
+        HS_REQUIRE(prec_kind == HS_PCG_NONE || prec, HS_EINVAL, "NULL preconditioner");
+        HS_REQUIRE(prec_kind >= HS_PCG_NONE && prec_kind <= HS_PCG_CHEB, HS_EINVAL,
+                   "unknown preconditioner kind");
+        const int64_t n = A->n;
+        HS_REQUIRE(n > 0, HS_EINVAL, "empty system");
+        if (prec_kind == HS_PCG_L1GS)
+            HS_REQUIRE(l1gs_rows(*reinterpret_cast<L1GSPlan *>(prec)) == n, HS_EINVAL,
+                       "preconditioner size does not match the matrix");
+        if (prec_kind == HS_PCG_CHEB)
+            HS_REQUIRE(cheb_rows(*reinterpret_cast<ChebPlan *>(prec)) == n, HS_EINVAL,
+                       "preconditioner size does not match the matrix");
+        hipStream_t user = as_stream(stream);
+        CGWork &w = work_for(*A);
+        PrecRef M{prec_kind, prec};
+        if (itmax < 0) itmax = 2 * n;
+        const int block = n < 200000 ? 16 : 4;
+        if (loc == HS_LOC_DEVICE) {
+            pcg_device(*A, M, w, b, x, rtol, atol, itmax, block, true, user, stats);
+        } else {
+            w.ensure(n);
+            HS_CHECK_HIP(hipMemcpyAsync(w.b.get(), b, n * 8, hipMemcpyHostToDevice, user));
+            DBuf<double> xd(n);
+            pcg_device(*A, M, w, w.b.get(), xd.get(), rtol, atol, itmax, block, true, user, stats);
+            HS_CHECK_HIP(hipMemcpyAsync(x, xd.get(), n * 8, hipMemcpyDeviceToHost, user));
+            HS_CHECK_HIP(hipStreamSynchronize(user));
+        }
+    });
+}
+
+}  // extern "C"

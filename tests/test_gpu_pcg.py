@@ -1,0 +1,158 @@
+"""GPU PCG (through the C ABI) against the oracle's CG.
+
+Parity bar: identical iteration counts to the FP64 oracle at rtol 1e-8, and
+solutions within 1e-9 relative (the dot products are summed in a different
+order, so iterates drift by rounding); the FP64 true relative residual is
+reported by the device and checked against a host recomputation.
+Mixed precision (FP32/BF16 smoothers inside FP64 CG) is held to the same
+final true residual, with its iteration-count delta bounded.
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _true_relres(A, b, x):
+    return float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+
+
+def test_spmv(gpu):
+    from hybridsmoothers import problems as pb
+    import oracle as O
+    hs = gpu
+    A = pb.lognormal3d(20, seed=1)
+    M = hs.DeviceMatrix(A)
+    x = np.random.default_rng(0).uniform(-1, 1, A.shape[0])
+    y = np.empty_like(x)
+    M.mul(y, x)
+    np.testing.assert_array_equal(y, O.spmv(O.csr1(A), x))
+    Mr = hs.DeviceMatrix(A.tocsr())
+    Mr.mul(y, x)
+    np.testing.assert_array_equal(y, O.spmv(O.csr1(A), x))
+
+
+def test_unpreconditioned(gpu):
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.poisson3d(24)
+    b = pb.uniform_rhs(A.shape[0], 0)
+    xo, so = O.pcg(A, b, None, rtol=1e-8)
+    sol = hs.solve(hs.LinearProblem(A, b), hs.KrylovJL_CG(), reltol=1e-8)
+    assert sol.status == "solved"
+    assert sol.iters == so.iters
+    assert relerr(sol.u, xo) < 1e-9
+    assert abs(sol.true_relres - _true_relres(A, b, sol.u)) < 1e-12
+
+
+def test_c1_readme_quickstart(gpu):
+    """README.md:17-30: 1D Laplacian N=2048, b = ones, L1GS P=16 symmetric, CG."""
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.laplace1d(2048)
+    b = np.ones(2048)
+    P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=20)(
+        A, 16, isSymA=True, sweep=hs.SymmetricSweep())
+    ref = O.l1gs_build(A, 16, is_sym=True, sweep="symmetric", gpu=(256, 20))
+    xo, so = O.pcg(A, b, ref, rtol=1e-8)
+    sol = hs.solve(hs.LinearProblem(A, b), hs.KrylovJL_CG(), Pl=P, reltol=1e-8)
+    assert sol.status == "solved"
+    assert sol.iters == so.iters
+    assert relerr(sol.u, xo) < 1e-9
+    assert sol.true_relres < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["l1gs64", "l1gs32_fwd", "cheb4_3", "cheb1_2", "cheb4_1"])
+def test_preconditioned_parity(gpu, prec):
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.lognormal3d(24, seed=7)
+    n = A.shape[0]
+    b = pb.uniform_rhs(n, 3)
+    if prec.startswith("l1gs"):
+        psz = 64 if prec == "l1gs64" else 32
+        sw = "symmetric"
+        P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=64)(A, psz, isSymA=True)
+        ref = O.l1gs_build(A, psz, is_sym=True, sweep=sw, gpu=(256, 64))
+    else:
+        kind, deg = (4 if prec.startswith("cheb4") else 1), int(prec[-1])
+        C = hs.ChebyshevFourth if kind == 4 else hs.ChebyshevFirst
+        P = hs._build_smoother(C(degree=deg, backend=hs.ROCBackend()), A)
+        ref = O.cheb_build(A, kind, deg)
+        ref.rho = P.rho
+    xo, so = O.pcg(A, b, ref, rtol=1e-8)
+    for rep in range(2):  # second solve replays the cached graph
+        sol = hs.solve(hs.LinearProblem(A, b), hs.KrylovJL_CG(), Pl=P, reltol=1e-8)
+        assert sol.status == "solved"
+        assert sol.iters == so.iters, (sol.iters, so.iters)
+        assert relerr(sol.u, xo) < 1e-9
+        assert abs(sol.true_relres - _true_relres(A, b, sol.u)) < 1e-12
+
+
+def test_device_vectors_and_reuse(gpu):
+    import torch
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.poisson3d(20)
+    n = A.shape[0]
+    M = hs.DeviceMatrix(A)
+    P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=64)(A, 32, isSymA=True)
+    sols = []
+    for seed in (1, 2, 1):
+        b = pb.uniform_rhs(n, seed)
+        bd = torch.tensor(b, device="cuda")
+        s = hs.pcg_solve(M, bd, Pl=P, reltol=1e-8)
+        sh = hs.pcg_solve(M, b, Pl=P, reltol=1e-8)
+        np.testing.assert_array_equal(s.u.cpu().numpy(), sh.u)
+        sols.append(sh)
+    np.testing.assert_array_equal(sols[0].u, sols[2].u)  # deterministic
+    assert sols[0].iters == sols[2].iters
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_mixed_precision_smoother(gpu, precision):
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.lognormal3d(32, seed=2)
+    n = A.shape[0]
+    b = pb.uniform_rhs(n, 2)
+    ref = O.l1gs_build(A, 64, is_sym=True, sweep="symmetric", gpu=(256, 64))
+    _, so = O.pcg(A, b, ref, rtol=1e-8)
+    P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=64)(
+        A, 64, isSymA=True, precision=precision)
+    sol = hs.solve(hs.LinearProblem(A, b), hs.KrylovJL_CG(), Pl=P, reltol=1e-8)
+    assert sol.status == "solved"
+    # FP64 outer CG: the final true residual matches the all-FP64 solve's scale
+    assert sol.true_relres < 1e-7
+    assert abs(sol.iters - so.iters) <= max(3, so.iters // 10), (sol.iters, so.iters)
+    C32 = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend(),
+                                                precision="fp32"), A)
+    ref4 = O.cheb_build(A, 4, 3)
+    _, s4 = O.pcg(A, b, ref4, rtol=1e-8)
+    sol4 = hs.solve(hs.LinearProblem(A, b), hs.KrylovJL_CG(), Pl=C32, reltol=1e-8)
+    assert sol4.status == "solved" and sol4.true_relres < 1e-7
+    assert abs(sol4.iters - s4.iters) <= max(3, s4.iters // 10)
+
+
+def test_c2_poisson128_l1gs_pcg(gpu):
+    """128^3 Poisson, L1GS P=64 symmetric as Pl, CG to rtol 1e-8: iteration parity."""
+    import torch
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.poisson3d(128)
+    n = A.shape[0]
+    b = pb.uniform_rhs(n, 0)
+    ref = O.l1gs_build(A, 64, is_sym=True, sweep="symmetric", gpu=(256, 1024))
+    xo, so = O.pcg(A, b, ref, rtol=1e-8)
+    P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=1024)(A, 64, isSymA=True)
+    sol = hs.pcg_solve(A, torch.tensor(b, device="cuda"), Pl=P, reltol=1e-8)
+    assert sol.status == "solved"
+    assert sol.iters == so.iters
+    assert relerr(sol.u.cpu().numpy(), xo) < 1e-8
