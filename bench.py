@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline sweep only")
     ap.add_argument("--grid2d", type=int, default=2048)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB per step (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the smoother path has no multi-GPU code "
@@ -97,6 +99,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        ms = time_kernel(lambda: dst.copy_(src), args.steps, args.warmup,
+                         torch.cuda.current_stream())
+        print(json.dumps({"copy_bytes_each_way": nb, "ms": ms,
+                          "GB/s_read+write": 2 * nb / (ms * 1e-3) / 1e9}), flush=True)
+        return
     import hybridsmoothers as hs
     from hybridsmoothers import problems as pb
 

@@ -1,0 +1,175 @@
+"""CPU-side checks of the C ABI and the host API (no GPU compute calls).
+
+* libhsmooth.so loads and exports every function include/hsmooth.h declares;
+* argument validation that happens before any device work returns the
+  reference's errors (partsize, threads, blocks; unknown enums);
+* the Python mirror rejects what the reference rejects, and reports a
+  non-functional backend when no GPU is visible;
+* the problem generators produce the documented matrices.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hsmooth.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(hs_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_exports_every_declared_symbol():
+    from hybridsmoothers import _lib as L
+    names = _declared()
+    assert len(names) >= 20
+    missing = [s for s in names if not hasattr(L.lib, s)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    exported = set(re.findall(r"\b(hs_[a-z0-9_]+)\b", out))
+    assert set(names) <= exported
+    assert set(L.EXPORTED) <= set(names)
+
+
+def test_library_is_gfx950_only():
+    """Every code object bundled in libhsmooth.so targets gfx950."""
+    from hybridsmoothers import _lib as L
+    data = open(L.LIB_PATH, "rb").read()
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa--(gfx[0-9a-z]+)", data))
+    assert targets == {b"gfx950"}
+
+
+def _build_args(n=9, partsize=2, threads=4, blocks=4, sweep=2, cache=0, prec=0, fmt=0):
+    from hybridsmoothers import problems as pb
+    A = pb.poisson_test_matrix(n)
+    ptr = A.indptr.astype(np.int64) + 1
+    ind = A.indices.astype(np.int64) + 1
+    val = A.data.astype(np.float64)
+    return (n, ptr, ind, val, fmt, 1, partsize, 1.5, sweep, cache, prec, threads, blocks)
+
+
+def _call_build(args):
+    from hybridsmoothers import _lib as L
+    n, ptr, ind, val, fmt, sym, P, eta, sw, cache, prec, th, bl = args
+    h = C.c_void_p()
+    st = L.lib.hs_l1gs_build(n, ptr.ctypes.data, ind.ctypes.data, val.ctypes.data, fmt, sym, P,
+                             eta, sw, cache, prec, th, bl, None, C.byref(h))
+    return st, L.lib.hs_last_error().decode()
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(partsize=0), "partsize must be greater than 0"),
+    (dict(threads=0), "`threads` must be greater than 0"),
+    (dict(blocks=-1), "`blocks` must be greater than 0"),
+    (dict(sweep=7), "unknown sweep"),
+    (dict(cache=3), "unknown cache strategy"),
+    (dict(prec=9), "unknown precision"),
+    (dict(fmt=5), "unknown matrix format"),
+])
+def test_build_argument_errors(kw, msg):
+    from hybridsmoothers import _lib as L
+    st, err = _call_build(_build_args(**kw))
+    assert st == L.HS_EINVAL
+    assert msg in err
+
+
+def test_matrix_view_precision_unsupported():
+    from hybridsmoothers import _lib as L
+    st, err = _call_build(_build_args(cache=1, prec=1))
+    assert st == L.HS_EUNSUPPORTED
+
+
+def test_null_handles():
+    from hybridsmoothers import _lib as L
+    assert L.lib.hs_l1gs_apply(None, None, None, 0, None, 0) == L.HS_EINVAL
+    assert L.lib.hs_l1gs_info(None, None) == L.HS_EINVAL
+    assert L.lib.hs_cheb_smooth(None, None, None, 0, None, 0) == L.HS_EINVAL
+    assert L.lib.hs_csr_spmv(None, None, None, 0, None, 0) == L.HS_EINVAL
+    assert L.lib.hs_l1gs_destroy(None) == L.HS_OK
+    h = C.c_void_p()
+    assert L.lib.hs_cheb_build(5, None, None, None, 3, 2, 50, 0, None, C.byref(h)) == L.HS_EINVAL
+    assert "kind must be 1" in L.lib.hs_last_error().decode()
+
+
+def _has_gpu():
+    from hybridsmoothers import l1gs
+    return l1gs.functional(l1gs.ROCBackend())
+
+
+def test_configs_and_backends():
+    import hybridsmoothers as hs
+    with pytest.raises(hs.HSError, match="`threads` must be greater than 0"):
+        hs.GPUConfig(0, 4)
+    with pytest.raises(hs.HSError, match="`blocks` must be greater than 0"):
+        hs.GPUConfig(4, 0)
+    with pytest.raises(hs.HSError, match="`chunks` must be greater than 0"):
+        hs.CPUConfig(0)
+    with pytest.raises(hs.HSError, match="KA.CPU"):
+        hs.L1GSPrecBuilder(hs.CPU(), chunks=4)
+    with pytest.raises(TypeError):
+        hs.L1GSPrecBuilder(hs.ROCBackend(), threads=4)
+    if not _has_gpu():
+        with pytest.raises(hs.HSError, match="not functional"):
+            hs.L1GSPrecBuilder(hs.ROCBackend(), threads=4, blocks=4)
+        with pytest.raises(hs.HSError, match="not functional"):
+            hs._build_smoother(hs.ChebyshevFourth(degree=2), sp.identity(4, format="csc"))
+
+
+def test_vector_checks():
+    from hybridsmoothers import _lib as L
+    with pytest.raises(L.HSError):
+        L.vec_ptr(np.zeros(5, np.float32), 5)
+    with pytest.raises(L.HSError):
+        L.vec_ptr(np.zeros(4), 5)
+    p, loc = L.vec_ptr(np.zeros(5), 5)
+    assert loc == L.LOC_HOST and p
+
+
+def test_sparse_arrays_one_based():
+    from hybridsmoothers.l1gs import sparse_arrays
+    from hybridsmoothers import problems as pb
+    A = pb.poisson_test_matrix(6)
+    n, ptr, ind, val, fmt = sparse_arrays(A)
+    assert n == 6 and fmt == 0 and ptr.dtype == np.int64 and ptr[0] == 1 and ind.min() == 1
+    n, ptr, ind, val, fmt = sparse_arrays(A.tocsr())
+    assert fmt == 1
+    with pytest.raises(Exception):
+        sparse_arrays(sp.csc_matrix(np.ones((3, 4))))
+
+
+def test_problem_generators():
+    from hybridsmoothers import problems as pb
+    A = pb.poisson3d(8)
+    assert A.shape == (512, 512) and (A != A.T).nnz == 0
+    K = sp.kron
+    T = pb.poisson_test_matrix(8)
+    I = sp.identity(8)
+    ref = (K(I, K(I, T)) + K(I, K(T, I)) + K(T, K(I, I))).tocsc()
+    assert abs(A - ref).max() == 0
+    # 128^3: 7 n^3 - 6 n^2 stored entries
+    n = 128
+    assert 7 * n ** 3 - 6 * n ** 2 == 14_581_760
+    C3 = pb.convdiff2d(16)
+    D = C3.diagonal()
+    assert np.all(D == 19.0)
+    row = C3.tocsr()[16 * 5 + 5].toarray().ravel()
+    assert row[16 * 5 + 4] == -11 and row[16 * 5 + 6] == -1
+    assert row[16 * 4 + 5] == -6 and row[16 * 6 + 5] == -1
+    # 2048^2 nnz: 5 n^2 - 4 n
+    assert 5 * 2048 ** 2 - 4 * 2048 == 20_963_328
+    L4 = pb.lognormal3d(6, seed=2)
+    assert (abs(L4 - L4.T) > 0).nnz == 0
+    assert np.all(np.linalg.eigvalsh(L4.toarray()) > 0)
+    L2 = pb.lognormal2d(9, seed=1)
+    assert (abs(L2 - L2.T) > 0).nnz == 0
+    specs = pb.serving_stream(64, seed=5)
+    sizes = [g * g if k == "2d" else g ** 3 for k, g, _ in specs]
+    assert min(sizes) >= 700 and max(sizes) <= 3.0e5
