@@ -128,14 +128,22 @@ def main():
         hs.ldiv_(y, prec, x, sync=False)
 
     log("[bench] timing symmetric sweep")
-    ms = time_kernel(step, args.steps, args.warmup, stream)
     nbytes = sweep_bytes(n, S)
+    kernel_variants = {}
+    for v in (0, 1, 2):
+        prec.set_variant(v)
+        mv = time_kernel(step, args.steps, args.warmup, stream)
+        kernel_variants[str(v)] = round(mv, 5)
+    best = min(kernel_variants, key=kernel_variants.get)
+    prec.set_variant(int(best))
+    ms = time_kernel(step, args.steps, args.warmup, stream)
     gbs = nbytes / (ms * 1e-3) / 1e9
     variants = []
     if not args.quick:
         # reduced-precision smoothers on the same matrix (FP64 x in, FP64 y out)
         for pname, tb, db in (("fp32", 4, 4), ("bf16", 2, 4)):
             pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname)
+            pv.set_variant(int(best))
             m = time_kernel(lambda: hs.ldiv_(y, pv, x, sync=False), args.steps, args.warmup,
                             stream)
             b = sweep_bytes(n, S, tri_bytes=tb, d_bytes=db)
@@ -178,6 +186,8 @@ def main():
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": nbytes},
         "sweeps": variants,
+        "kernel_variant_ms": kernel_variants,
+        "kernel_variant": int(best),
     }
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")

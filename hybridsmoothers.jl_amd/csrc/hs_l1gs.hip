@@ -6,13 +6,20 @@
 // Apply: forward, backward and symmetric sweeps, one partition per wave
 // segment (reference: _apply_sweep_kernel! :1068-1095, _apply_sweep! :846-906).
 //
-// Device layout of the packed triangles.  Block k (0-based) starts at
-// k*P*(P-1)/2 like the reference's packed buffers (:698-701), but inside a block
-// the triangles are stored COLUMN-major so that step j of the right-looking
-// recurrence reads one contiguous run of values, one per lane:
-//   L(i,j), j < i : boff + j*(Pa-1) - j*(j-1)/2 + (i-j-1)
-//   U(i,j), i < j : boff + j*(j-1)/2 + i
-// (Pa = rows of this partition; only the last partition can be shorter.)
+// Device layout of the packed triangles: ONE buffer, one block per partition,
+// holding the triangles the sweep reads in the order it reads them.  Block k
+// starts at k*P*(P-1) for a symmetric sweep (lower then upper triangle) and at
+// k*P*(P-1)/2 for a forward (lower only) or backward (upper only) sweep, like
+// the reference's packed buffers (:698-701) laid end to end.  Inside a block
+// the triangles are COLUMN-major, so that step j of the right-looking
+// recurrence reads one contiguous run of values, one per lane, and the upper
+// triangle's columns are stored in descending order, the order the backward
+// recurrence visits them.  A symmetric sweep therefore streams each block once,
+// front to back:
+//   L(i,j), j < i : lb + j*(Pa-1) - j*(j-1)/2 + (i-j-1)
+//   U(i,j), i < j : ub + (Pa*(Pa-1) - j*(j+1))/2 + i
+// with lb = block start and ub = lb + Pa*(Pa-1)/2 (symmetric) or lb (backward);
+// Pa = rows of this partition (only the last partition can be shorter).
 // The reference row-major layout is produced on request by hs_l1gs_get.
 //
 // Recurrence.  Lane i of a segment owns row i.  Forward step j: every lane
@@ -26,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 namespace hs {
@@ -38,7 +46,19 @@ enum { MODE_FWD = 0, MODE_BWD = 1, MODE_SYM = 2 };
 __host__ __device__ __forceinline__ int64_t colL_off(int64_t j, int64_t Pa) {
     return j * (Pa - 1) - (j * (j - 1)) / 2;
 }
-__host__ __device__ __forceinline__ int64_t colU_off(int64_t j) { return (j * (j - 1)) / 2; }
+__host__ __device__ __forceinline__ int64_t colU_off(int64_t j, int64_t Pa) {
+    return (Pa * (Pa - 1) - j * (j + 1)) / 2;
+}
+// elements per partition block of the device buffer
+__host__ __device__ __forceinline__ int64_t blk_stride(int64_t P, int mode) {
+    return mode == MODE_SYM ? P * (P - 1) : (P * (P - 1)) / 2;
+}
+// start of the lower and upper triangles of partition k (Pa rows)
+__host__ __device__ __forceinline__ void block_bases(int64_t k, int64_t P, int64_t Pa, int mode,
+                                                     int64_t &lb, int64_t &ub) {
+    lb = k * blk_stride(P, mode);
+    ub = (mode == MODE_SYM) ? lb + (Pa * (Pa - 1)) / 2 : lb;
+}
 
 __host__ __device__ __forceinline__ double dl1_rule(double a, double d, double eta) {
     // _compute_D_Dl1_kernel! (:1016-1022): a_ii + (a_ii >= eta*d ? 0 : d/2)
@@ -56,8 +76,8 @@ __host__ __device__ __forceinline__ double dl1_rule(double a, double d, double e
 template <class TS>
 __global__ void k_setup_rows(int64_t n, int64_t P, const int64_t *__restrict__ ptr,
                              const int64_t *__restrict__ ind, const double *__restrict__ val,
-                             double eta, double *__restrict__ D, TS *__restrict__ Lc,
-                             TS *__restrict__ Uc, int want_L, int want_U) {
+                             double eta, double *__restrict__ D, TS *__restrict__ T, int mode,
+                             int want_L, int want_U) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     int64_t k = g / P;
@@ -65,7 +85,8 @@ __global__ void k_setup_rows(int64_t n, int64_t P, const int64_t *__restrict__ p
     int64_t e = (s + P < n ? s + P : n) - 1;
     int64_t Pa = e - s + 1;
     int64_t i = g - s;
-    int64_t boff = k * ((P * (P - 1)) / 2);
+    int64_t lb, ub;
+    block_bases(k, P, Pa, mode, lb, ub);
     double a = 0.0, d = 0.0;
     int64_t p0 = ptr[g] - 1, p1 = ptr[g + 1] - 1;
     for (int64_t p = p0; p < p1; ++p) {
@@ -78,12 +99,12 @@ __global__ void k_setup_rows(int64_t n, int64_t P, const int64_t *__restrict__ p
         } else if (j < g) {
             if (want_L) {
                 int64_t jl = j - s;
-                Lc[boff + colL_off(jl, Pa) + (i - jl - 1)] = Scalar<TS>::from_f64(v);
+                T[lb + colL_off(jl, Pa) + (i - jl - 1)] = Scalar<TS>::from_f64(v);
             }
         } else {
             if (want_U) {
                 int64_t jl = j - s;
-                Uc[boff + colU_off(jl) + i] = Scalar<TS>::from_f64(v);
+                T[ub + colU_off(jl, Pa) + i] = Scalar<TS>::from_f64(v);
             }
         }
     }
@@ -96,7 +117,7 @@ __global__ void k_setup_rows(int64_t n, int64_t P, const int64_t *__restrict__ p
 template <class TS>
 __global__ void k_pack_cols(int64_t n, int64_t P, const int64_t *__restrict__ colptr,
                             const int64_t *__restrict__ rowval, const double *__restrict__ val,
-                            TS *__restrict__ Lc, TS *__restrict__ Uc, int want_L, int want_U) {
+                            TS *__restrict__ T, int mode, int want_L, int want_U) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     int64_t k = c / P;
@@ -104,16 +125,17 @@ __global__ void k_pack_cols(int64_t n, int64_t P, const int64_t *__restrict__ co
     int64_t e = (s + P < n ? s + P : n) - 1;
     int64_t Pa = e - s + 1;
     int64_t jl = c - s;
-    int64_t boff = k * ((P * (P - 1)) / 2);
+    int64_t lb, ub;
+    block_bases(k, P, Pa, mode, lb, ub);
     int64_t p0 = colptr[c] - 1, p1 = colptr[c + 1] - 1;
     for (int64_t p = p0; p < p1; ++p) {
         int64_t r = rowval[p] - 1;
         if (r < s) continue;
         if (r > e) break;
         if (r > c) {
-            if (want_L) Lc[boff + colL_off(jl, Pa) + (r - c - 1)] = Scalar<TS>::from_f64(val[p]);
+            if (want_L) T[lb + colL_off(jl, Pa) + (r - c - 1)] = Scalar<TS>::from_f64(val[p]);
         } else if (r < c) {
-            if (want_U) Uc[boff + colU_off(jl) + (r - s)] = Scalar<TS>::from_f64(val[p]);
+            if (want_U) T[ub + colU_off(jl, Pa) + (r - s)] = Scalar<TS>::from_f64(val[p]);
         }
     }
 }
@@ -297,7 +319,7 @@ __device__ __forceinline__ TC sweep_generic(TC xr, TC dr, TC rr, int lane, int P
                 yfin = (lane == j) ? q : yfin;
                 if (j > 0) {
                     TC yj = seg_bcast<PM>(q, j);
-                    TC u = (lane < j) ? load_as<TC>(Uk[colU_off(j) + lane]) : (TC)0;
+                    TC u = (lane < j) ? load_as<TC>(Uk[colU_off(j, Pa) + lane]) : (TC)0;
                     if (lane < j) acc = acc + u * yj;
                 }
             }
@@ -320,10 +342,10 @@ __device__ __forceinline__ TC sweep_full(TC xr, TC dr, TC rr, int lane,
     constexpr int R = RING < NC ? RING : NC;
     // Unconditional loads: lanes whose row is already final (i <= j forward,
     // i >= j backward) read a neighbouring in-block value into an accumulator
-    // that is never used again.  Indices stay inside block k: forward column j
-    // reads colL(j) + lane - j - 1 >= 0 for j >= 1 (clamped for j = 0);
-    // backward column j reads colU(j) + lane <= P(P-1)/2 - 1 for j <= P-2
-    // (clamped for j = P-1).
+    // that is never used again.  Indices stay inside the triangle: forward
+    // column j reads colL(j) + lane - j - 1 >= 0 for j >= 1 (clamped for j = 0);
+    // backward column j reads colU(j) + lane, clamped to the column for the
+    // last stored columns (small j), whose tail would pass the triangle's end.
     auto load_col = [&](int c) -> TS {
         if (c < NL) {
             const int j = c;
@@ -331,8 +353,10 @@ __device__ __forceinline__ TC sweep_full(TC xr, TC dr, TC rr, int lane,
             return Lk[colL_off(j, PM) + off];
         }
         const int j = PM - 1 - (c - NL);
-        const int off = (j == PM - 1) ? (lane < PM - 2 ? lane : PM - 2) : lane;
-        return Uk[colU_off(j) + off];
+        constexpr int64_t UEND = (PM * (PM - 1)) / 2;
+        const bool tail = colU_off(j, PM) + PM > UEND;
+        const int off = tail ? (lane < j - 1 ? lane : j - 1) : lane;
+        return Uk[colU_off(j, PM) + off];
     };
     TS ring[R > 0 ? R : 1];
 #pragma unroll
@@ -371,13 +395,244 @@ __device__ __forceinline__ TC sweep_full(TC xr, TC dr, TC rr, int lane,
     return yr;
 }
 
+// ---------------------------------------------------------------------------
+// P = 64 symmetric sweep streamed through LDS.
+//
+// A full partition's block is 64*63 = 4032 values = 63 chunks of 64 values,
+// the lower triangle's columns then the upper triangle's (descending).  The
+// wave copies chunk c with one aligned full-wave load (lane i takes value
+// 64c + i) into a ring of NR chunks in LDS, RUN steps before the first step
+// that reads it, and every recurrence step reads its column back from LDS
+// (one value per lane, consecutive addresses).  Compared with loading each
+// column straight from global memory this halves the vector-memory
+// instructions and makes every one of them a 512-byte aligned transfer; the
+// whole schedule is fixed at compile time.
+// ---------------------------------------------------------------------------
+namespace ring64 {
+constexpr int P = 64;
+constexpr int NSTEP = 2 * (P - 1);          // 63 forward + 63 backward columns
+constexpr int LTRI = P * (P - 1) / 2;       // 2016
+constexpr int NCHUNK = P - 1;               // 4032 / 64
+constexpr int colL(int j) { return j * (P - 1) - j * (j - 1) / 2; }
+constexpr int colU(int j) { return (P * (P - 1) - j * (j + 1)) / 2; }
+// column read at step t and the block offset of lane 0's value
+constexpr int step_col(int t) { return t < P - 1 ? t : (P - 1) - (t - (P - 1)); }
+constexpr int step_base(int t) {
+    return t < P - 1 ? colL(t) - t - 1 : LTRI + colU(step_col(t));
+}
+// live values of step t: [lo, hi)
+constexpr int step_lo(int t) { return t < P - 1 ? colL(t) : LTRI + colU(step_col(t)); }
+constexpr int step_hi(int t) {
+    return t < P - 1 ? colL(t + 1) : LTRI + colU(step_col(t)) + step_col(t);
+}
+constexpr int first_use(int c) {
+    for (int t = 0; t < NSTEP; ++t)
+        if (step_lo(t) < 64 * (c + 1) && step_hi(t) > 64 * c) return t;
+    return NSTEP;
+}
+constexpr int last_use(int c) {
+    int last = -1;
+    for (int t = 0; t < NSTEP; ++t)
+        if (step_lo(t) < 64 * (c + 1) && step_hi(t) > 64 * c) last = t;
+    return last;
+}
+// Ring slots must not be rewritten while an older chunk is still live.  A
+// chunk first read at step t is written to LDS while the group of G steps
+// before t's group computes, after that earlier group's columns were read
+// into registers: the older chunk of the slot must not be read at or after
+// the start of the writing group's successor, i.e. last use < G*floor(t/G).
+template <int NR, int G>
+constexpr bool ring_ok() {
+    for (int c = NR; c < NCHUNK; ++c)
+        if (last_use(c - NR) >= (first_use(c) / G) * G) return false;
+    return true;
+}
+}  // namespace ring64
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+namespace ring64 {
+struct Sched {
+    int first[NCHUNK];  // first step reading chunk c
+    int issue[NCHUNK];  // step whose group issues the load of chunk c (0: prologue)
+};
+constexpr Sched make_sched(int run) {
+    Sched sc{};
+    for (int c = 0; c < NCHUNK; ++c) {
+        sc.first[c] = first_use(c);
+        int is = first_use(c) - run;
+        sc.issue[c] = is < 0 ? 0 : is;
+    }
+    return sc;
+}
+// k-th chunk (ascending) whose first use, or whose issue step, is t; -1 if none
+constexpr int kth_first(const Sched &sc, int t, int k) {
+    for (int c = 0; c < NCHUNK; ++c)
+        if (sc.first[c] == t && k-- == 0) return c;
+    return -1;
+}
+constexpr int kth_issue(const Sched &sc, int t, int k) {
+    for (int c = 0; c < NCHUNK; ++c)
+        if (sc.issue[c] == t && k-- == 0) return c;
+    return -1;
+}
+constexpr int max_per_step(const Sched &sc) {
+    int m = 0;
+    for (int t = 1; t < NSTEP; ++t) {
+        int a = 0, b = 0;
+        for (int c = 0; c < NCHUNK; ++c) {
+            a += sc.first[c] == t;
+            b += sc.issue[c] == t;
+        }
+        m = a > m ? a : m;
+        m = b > m ? b : m;
+    }
+    return m;
+}
+constexpr int count_issue0(const Sched &sc) {
+    int a = 0;
+    for (int c = 0; c < NCHUNK; ++c) a += sc.issue[c] == 0;
+    return a;
+}
+}  // namespace ring64
+
+template <class TS, class TC, int NR, int RUN>
+__device__ __forceinline__ TC sweep_ring64(TC xr, TC dr, TC rr, int lane,
+                                           const TS *__restrict__ blk, TS *ring) {
+    using namespace ring64;
+    constexpr int G = 6;                       // steps per register group
+    static_assert(ring_ok<NR, G>(), "LDS ring too small for the chunk schedule");
+    static_assert((NR & (NR - 1)) == 0, "ring size must be a power of two");
+    constexpr Sched SC = make_sched(RUN);
+    static_assert(max_per_step(SC) <= 2, "more than two chunks scheduled in one step");
+    constexpr int NG = (NSTEP + G - 1) / G;
+    constexpr int RING = NR * 64;
+    TS chunk[NCHUNK];                          // in-flight chunk registers
+    TS col[2][G];                              // column values of two groups
+
+    // global loads issued at step t >= 1 (first use RUN steps later)
+    auto issue_at = [&](auto T) {
+        constexpr int t = decltype(T)::value;
+        constexpr int c0 = kth_issue(SC, t, 0), c1 = kth_issue(SC, t, 1);
+        if constexpr (c0 >= 0) chunk[c0] = blk[64 * c0 + lane];
+        if constexpr (c1 >= 0) chunk[c1] = blk[64 * c1 + lane];
+    };
+    // LDS writes of the chunks first read at step t
+    auto write_at = [&](auto T) {
+        constexpr int t = decltype(T)::value;
+        constexpr int c0 = kth_first(SC, t, 0), c1 = kth_first(SC, t, 1);
+        if constexpr (c0 >= 0) ring[(c0 % NR) * 64 + lane] = chunk[c0];
+        if constexpr (c1 >= 0) ring[(c1 % NR) * 64 + lane] = chunk[c1];
+    };
+    auto read_at = [&](auto T) -> TS {
+        constexpr int t = decltype(T)::value;
+        constexpr int b = step_base(t);
+        constexpr int m = ((b % RING) + RING) % RING;
+        if constexpr (m + 63 < RING) return ring[m + lane];  // immediate offset
+        else return ring[(lane + m) & (RING - 1)];
+    };
+
+    // Lanes exchange values through LDS: a lane reads what other lanes wrote.
+    // From one thread's view ring[a + lane] and ring[b + lane] never alias, so
+    // without a fence the compiler may reorder a write and a read; a
+    // wavefront-scope fence orders them (the LDS itself executes a wave's
+    // operations in order, so no wait instruction is needed).
+    auto lds_fence = [] { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); };
+
+    // prologue: the loads first used within RUN steps, group 0 into LDS and registers
+    static_for<0, count_issue0(SC)>([&](auto K) {
+        constexpr int c = kth_issue(SC, 0, decltype(K)::value);
+        chunk[c] = blk[64 * c + lane];
+    });
+    static_for<0, G>([&](auto T) { write_at(T); });
+    lds_fence();
+    static_for<0, G>([&](auto T) { col[0][decltype(T)::value] = read_at(T); });
+
+    TC acc = (TC)0, yfin = (TC)0, ysym = xr;
+    static_for<0, NG>([&](auto Gi) {
+        constexpr int g = decltype(Gi)::value;
+        constexpr int t0 = g * G;
+        // keep each group's loads, LDS traffic and arithmetic in place: left
+        // alone the scheduler hoists every chunk load to the top
+        __builtin_amdgcn_sched_barrier(0);
+        static_for<t0 + 1, t0 + G + 1>([&](auto T) {
+            if constexpr (decltype(T)::value < NSTEP) issue_at(T);
+        });
+        lds_fence();  // earlier reads of the slots rewritten below
+        static_for<t0 + G, t0 + 2 * G>([&](auto T) {
+            if constexpr (decltype(T)::value < NSTEP) write_at(T);
+        });
+        lds_fence();  // the writes above before the reads below
+        static_for<t0 + G, t0 + 2 * G>([&](auto T) {
+            constexpr int t = decltype(T)::value;
+            if constexpr (t < NSTEP) col[(g + 1) & 1][t - t0 - G] = read_at(T);
+        });
+        static_for<t0, t0 + G>([&](auto T) {
+            constexpr int t = decltype(T)::value;
+            if constexpr (t < NSTEP) {
+                if constexpr (t == P - 1) {
+                    // last forward row, y .= D_Dl1 .* y (:897), then the backward pass
+                    TC q = quot_fast(xr - acc, dr, rr);
+                    TC yf = (lane == P - 1) ? q : yfin;
+                    ysym = dr * yf;
+                    acc = (TC)0;
+                    yfin = (TC)0;
+                }
+                const TC xin = (t < P - 1) ? xr : ysym;
+                constexpr int j = step_col(t);
+                TC q = quot_fast(xin - acc, dr, rr);
+                TC yj = bcast_keep<64>(q, j, yfin);
+                acc = acc + load_as<TC>(col[g & 1][t - t0]) * yj;  // dead for finished lanes
+            }
+        });
+    });
+    TC q = quot_fast(ysym - acc, dr, rr);
+    return (lane == 0) ? q : yfin;
+}
+
+template <class TS, class TC, int NR, int RUN>
+__global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
+                                                     const TS *__restrict__ T,
+                                                     const TC *__restrict__ D, int64_t n,
+                                                     int64_t nparts, const int *__restrict__ done) {
+    if (done && *done) return;
+    __shared__ TS ring[NR * 64];
+    const int lane = threadIdx.x;
+    const int64_t k = blockIdx.x;
+    if (k >= nparts) return;
+    const int64_t s = k * 64;
+    const int64_t rem = n - s;
+    const int Pa = (int)(rem < 64 ? rem : 64);
+    int64_t lb, ub;
+    block_bases(k, 64, Pa, MODE_SYM, lb, ub);
+    const bool live = lane < Pa;
+    const TC xr = live ? (TC)x[s + lane] : (TC)0;
+    const TC dr = live ? D[s + lane] : (TC)1;
+    const TC rr = (TC)1 / dr;
+    TC yr;
+    if (__any(!safe_divisor(dr))) {
+        yr = sweep_generic<64, MODE_SYM, false, TS, TC>(xr, dr, rr, lane, Pa, live, T + lb, T + ub);
+    } else if (Pa == 64) {
+        yr = sweep_ring64<TS, TC, NR, RUN>(xr, dr, rr, lane, T + lb, ring);
+    } else {
+        yr = sweep_generic<64, MODE_SYM, true, TS, TC>(xr, dr, rr, lane, Pa, live, T + lb, T + ub);
+    }
+    if (live) y[s + lane] = (double)yr;
+}
+
 // One partition per PM-lane segment, 256 threads per workgroup.
 // TS: storage type of the triangles, TC: arithmetic type (double or float).
 // x and y may alias (each partition reads its rows before writing them).
 template <int PM, int MODE, class TS, class TC>
 __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
-                                                   const TS *__restrict__ Lc,
-                                                   const TS *__restrict__ Uc,
+                                                   const TS *__restrict__ T,
                                                    const TC *__restrict__ D, int64_t n, int P,
                                                    int64_t nparts, const int *__restrict__ done) {
     if (done && *done) return;  // converged CG: speculative launches do nothing
@@ -388,10 +643,11 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
     const int64_t s = k * (int64_t)P;
     const int64_t rem = n - s;
     const int Pa = (int)(rem < P ? rem : P);
-    const int64_t boff = k * (((int64_t)P * (P - 1)) / 2);
+    int64_t lb, ub;
+    block_bases(k, P, Pa, MODE, lb, ub);
     const bool live = lane < Pa;
-    const TS *Lk = Lc + boff;
-    const TS *Uk = Uc + boff;
+    const TS *Lk = T + lb;
+    const TS *Uk = T + ub;
 
     const TC xr = live ? (TC)x[s + lane] : (TC)0;
     const TC dr = live ? D[s + lane] : (TC)1;
@@ -411,22 +667,23 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
 // summing in the reference's order (left-looking).  Correctness path for
 // large partsize (e.g. partsize = N in the Gauss-Seidel sanity checks).
 template <int MODE, class TS, class TC>
-__global__ void k_sweep_thread(const double *x, double *y, const TS *__restrict__ Lc,
-                               const TS *__restrict__ Uc, const TC *__restrict__ D, int64_t n,
-                               int64_t P, int64_t nparts, const int *__restrict__ done) {
+__global__ void k_sweep_thread(const double *x, double *y, const TS *__restrict__ T,
+                               const TC *__restrict__ D, int64_t n, int64_t P, int64_t nparts,
+                               const int *__restrict__ done) {
     if (done && *done) return;
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nparts) return;
     int64_t s = k * P;
     int64_t Pa = (n - s < P) ? n - s : P;
-    int64_t boff = k * ((P * (P - 1)) / 2);
+    int64_t lb, ub;
+    block_bases(k, P, Pa, MODE, lb, ub);
     if (x != y)
         for (int64_t i = 0; i < Pa; ++i) y[s + i] = x[s + i];
     if (MODE != MODE_BWD) {
         for (int64_t i = 0; i < Pa; ++i) {
             TC acc = 0;
             for (int64_t j = 0; j < i; ++j)
-                acc = acc + load_as<TC>(Lc[boff + colL_off(j, Pa) + (i - j - 1)]) * (TC)y[s + j];
+                acc = acc + load_as<TC>(T[lb + colL_off(j, Pa) + (i - j - 1)]) * (TC)y[s + j];
             y[s + i] = (double)(((TC)y[s + i] - acc) / D[s + i]);
         }
     }
@@ -436,7 +693,7 @@ __global__ void k_sweep_thread(const double *x, double *y, const TS *__restrict_
         for (int64_t i = Pa - 1; i >= 0; --i) {
             TC acc = 0;
             for (int64_t j = i + 1; j < Pa; ++j)
-                acc = acc + load_as<TC>(Uc[boff + colU_off(j) + i]) * (TC)y[s + j];
+                acc = acc + load_as<TC>(T[ub + colU_off(j, Pa) + i]) * (TC)y[s + j];
             y[s + i] = (double)(((TC)y[s + i] - acc) / D[s + i]);
         }
     }
@@ -520,9 +777,12 @@ struct L1GSPlan {
     int64_t n = 0, P = 1, nparts = 0, nchunks = 1, chunksize = 1, S = 0;
     int sweep = HS_SWEEP_SYMMETRIC, cache = HS_CACHE_PACKED, prec = HS_PREC_FP64;
     int fmt = HS_FMT_CSC, sym = 0, rowwise = 1;
+    // kernel variant of the P = 64 symmetric sweep: 0 column loads straight to
+    // registers, 1-2 LDS-ring configurations (see hs_l1gs_set_variant)
+    int variant = 2;
     DBuf<double> D;   // D_Dl1 in FP64 (always kept: getter and FP64 sweeps)
     DBuf<float> Df;   // FP32 copy for the reduced-precision sweeps
-    DBuf<uint8_t> L, U;  // packed triangles, element type by `prec`
+    DBuf<uint8_t> T;  // packed triangles (block layout above), element type by `prec`
     // MatrixViewCache: device copy of the matrix (1-based, as given)
     DBuf<int64_t> vptr, vind, vdiag;
     DBuf<double> vval;
@@ -536,21 +796,20 @@ static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *
                        const int *done) {
     constexpr int SEGS = 256 / PM;
     unsigned grid = grid_for(h.nparts, SEGS);
-    const TS *L = reinterpret_cast<const TS *>(h.L.get());
-    const TS *U = reinterpret_cast<const TS *>(h.U.get());
+    const TS *T = reinterpret_cast<const TS *>(h.T.get());
     int P = (int)h.P;
     switch (h.sweep) {
     case HS_SWEEP_FORWARD:
         hipLaunchKernelGGL((k_sweep_seg<PM, MODE_FWD, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
-                           L, U, D, h.n, P, h.nparts, done);
+                           T, D, h.n, P, h.nparts, done);
         break;
     case HS_SWEEP_BACKWARD:
         hipLaunchKernelGGL((k_sweep_seg<PM, MODE_BWD, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
-                           L, U, D, h.n, P, h.nparts, done);
+                           T, D, h.n, P, h.nparts, done);
         break;
     default:
         hipLaunchKernelGGL((k_sweep_seg<PM, MODE_SYM, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
-                           L, U, D, h.n, P, h.nparts, done);
+                           T, D, h.n, P, h.nparts, done);
         break;
     }
 }
@@ -558,24 +817,34 @@ static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *
 template <class TS, class TC>
 static void launch_packed(const L1GSPlan &h, const double *x, double *y, const TC *D,
                           hipStream_t st, const int *done) {
+    if (h.P == 64 && h.sweep == HS_SWEEP_SYMMETRIC && h.variant > 0) {
+        const TS *T = reinterpret_cast<const TS *>(h.T.get());
+        const dim3 grid(grid_for(h.nparts, 1)), block(64);
+        if (h.variant == 1)
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 8, 12>), grid, block, 0, st, x, y, T, D,
+                               h.n, h.nparts, done);
+        else
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 16, 24>), grid, block, 0, st, x, y, T, D,
+                               h.n, h.nparts, done);
+        return;
+    }
     if (h.P <= 4) launch_seg<4, TS, TC>(h, x, y, D, st, done);
     else if (h.P <= 8) launch_seg<8, TS, TC>(h, x, y, D, st, done);
     else if (h.P <= 16) launch_seg<16, TS, TC>(h, x, y, D, st, done);
     else if (h.P <= 32) launch_seg<32, TS, TC>(h, x, y, D, st, done);
     else if (h.P <= 64) launch_seg<64, TS, TC>(h, x, y, D, st, done);
     else {
-        const TS *L = reinterpret_cast<const TS *>(h.L.get());
-        const TS *U = reinterpret_cast<const TS *>(h.U.get());
+        const TS *T = reinterpret_cast<const TS *>(h.T.get());
         unsigned grid = grid_for(h.nparts, 64);
         if (h.sweep == HS_SWEEP_FORWARD)
             hipLaunchKernelGGL((k_sweep_thread<MODE_FWD, TS, TC>), dim3(grid), dim3(64), 0, st, x,
-                               y, L, U, D, h.n, h.P, h.nparts, done);
+                               y, T, D, h.n, h.P, h.nparts, done);
         else if (h.sweep == HS_SWEEP_BACKWARD)
             hipLaunchKernelGGL((k_sweep_thread<MODE_BWD, TS, TC>), dim3(grid), dim3(64), 0, st, x,
-                               y, L, U, D, h.n, h.P, h.nparts, done);
+                               y, T, D, h.n, h.P, h.nparts, done);
         else
             hipLaunchKernelGGL((k_sweep_thread<MODE_SYM, TS, TC>), dim3(grid), dim3(64), 0, st, x,
-                               y, L, U, D, h.n, h.P, h.nparts, done);
+                               y, T, D, h.n, h.P, h.nparts, done);
     }
 }
 
@@ -640,7 +909,7 @@ static void nonsym_D(L1GSPlan &h, const int64_t *h_ptr, const int64_t *h_ind, co
     }
     hipLaunchKernelGGL((k_setup_rows<double>), dim3(grid_for(n, 256)), dim3(256), 0, st, n, h.P,
                        trp.get(), tci.get(), tcv.get(), eta, h.D.get(), (double *)nullptr,
-                       (double *)nullptr, 0, 0);
+                       h.sweep, 0, 0);
     check_launch();
     HS_CHECK_HIP(hipStreamSynchronize(st));  // the host arrays above go out of scope
 }
@@ -654,28 +923,22 @@ static void build_packed(L1GSPlan &h, const DBuf<int64_t> &ptr, const DBuf<int64
                          const double *h_val, double eta, hipStream_t st) {
     const int want_L = h.sweep != HS_SWEEP_BACKWARD;
     const int want_U = h.sweep != HS_SWEEP_FORWARD;
-    if (want_L) {
-        h.L.alloc(h.S * sizeof(TS));
-        HS_CHECK_HIP(hipMemsetAsync(h.L.get(), 0, h.L.bytes(), st));
-    }
-    if (want_U) {
-        h.U.alloc(h.S * sizeof(TS));
-        HS_CHECK_HIP(hipMemsetAsync(h.U.get(), 0, h.U.bytes(), st));
-    }
-    TS *L = reinterpret_cast<TS *>(h.L.get());
-    TS *U = reinterpret_cast<TS *>(h.U.get());
+    const int64_t elems = h.S * (h.sweep == HS_SWEEP_SYMMETRIC ? 2 : 1);
+    h.T.alloc(elems * sizeof(TS));
+    if (elems) HS_CHECK_HIP(hipMemsetAsync(h.T.get(), 0, h.T.bytes(), st));
+    TS *T = reinterpret_cast<TS *>(h.T.get());
     unsigned grid = grid_for(h.n, 256);
     if (h.rowwise) {
         // one pass: D_Dl1 and both triangles (the reference launches its
         // packing kernel once per triangle and recomputes D_Dl1, :805-837)
         hipLaunchKernelGGL((k_setup_rows<TS>), dim3(grid), dim3(256), 0, st, h.n, h.P, ptr.get(),
-                           ind.get(), val.get(), eta, h.D.get(), L, U, want_L, want_U);
+                           ind.get(), val.get(), eta, h.D.get(), T, h.sweep, want_L, want_U);
         check_launch();
         return;
     }
     nonsym_D(h, h_ptr, h_ind, h_val, eta, st);
     hipLaunchKernelGGL((k_pack_cols<TS>), dim3(grid), dim3(256), 0, st, h.n, h.P, ptr.get(),
-                       ind.get(), val.get(), L, U, want_L, want_U);
+                       ind.get(), val.get(), T, h.sweep, want_L, want_U);
     check_launch();
 }
 
@@ -768,7 +1031,7 @@ int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const doubl
                 if (h->rowwise) {
                     hipLaunchKernelGGL((k_setup_rows<double>), dim3(grid_for(n, 256)), dim3(256),
                                        0, st, n, h->P, dptr.get(), dind.get(), dval.get(), eta,
-                                       h->D.get(), (double *)nullptr, (double *)nullptr, 0, 0);
+                                       h->D.get(), (double *)nullptr, h->sweep, 0, 0);
                     check_launch();
                 } else {
                     nonsym_D(*h, ptr, ind, val, eta, st);
@@ -837,17 +1100,17 @@ int hs_l1gs_get(hs_l1gs_t h, int what, double *out) {
         HS_REQUIRE(what == HS_GET_SL || what == HS_GET_SU, HS_EINVAL, "unknown field");
         HS_REQUIRE(h->cache == HS_CACHE_PACKED, HS_EINVAL,
                    "packed buffers exist only for PackedBufferCache");
-        const DBuf<uint8_t> &buf = (what == HS_GET_SL) ? h->L : h->U;
         const bool has = (what == HS_GET_SL) ? h->sweep != HS_SWEEP_BACKWARD
                                              : h->sweep != HS_SWEEP_FORWARD;
         HS_REQUIRE(has, HS_EINVAL,
                    what == HS_GET_SL ? "this sweep has no lower buffer"
                                      : "this sweep has no upper buffer");
-        std::vector<double> internal(h->S);
+        const int64_t elems = h->S * (h->sweep == HS_SWEEP_SYMMETRIC ? 2 : 1);
+        std::vector<double> internal(elems);
         const size_t es = h->elem();
-        std::vector<uint8_t> raw(h->S * es);
-        if (h->S) HS_CHECK_HIP(hipMemcpy(raw.data(), buf.get(), raw.size(), hipMemcpyDeviceToHost));
-        for (int64_t t = 0; t < h->S; ++t) {
+        std::vector<uint8_t> raw(elems * es);
+        if (elems) HS_CHECK_HIP(hipMemcpy(raw.data(), h->T.get(), raw.size(), hipMemcpyDeviceToHost));
+        for (int64_t t = 0; t < elems; ++t) {
             if (h->prec == HS_PREC_FP64) {
                 double v;
                 std::memcpy(&v, &raw[t * 8], 8);
@@ -865,22 +1128,32 @@ int hs_l1gs_get(hs_l1gs_t h, int what, double *out) {
                 internal[t] = v;
             }
         }
-        // column-major (device) -> row-major (reference :1264, :1398)
+        // device block layout -> reference row-major buffers (:1264, :1398)
         for (int64_t k = 0; k < h->nparts; ++k) {
             int64_t s = k * P;
             int64_t Pa = (n - s < P) ? n - s : P;
             int64_t boff = k * ((P * (P - 1)) / 2);
+            int64_t lb, ub;
+            block_bases(k, P, Pa, h->sweep, lb, ub);
             for (int64_t i = 0; i < Pa; ++i) {
                 if (what == HS_GET_SL) {
                     for (int64_t j = 0; j < i; ++j)
-                        out[boff + (i * (i - 1)) / 2 + j] = internal[boff + colL_off(j, Pa) + (i - j - 1)];
+                        out[boff + (i * (i - 1)) / 2 + j] = internal[lb + colL_off(j, Pa) + (i - j - 1)];
                 } else {
                     int64_t row_start = i * Pa - (i * (i + 1)) / 2;
                     for (int64_t j = i + 1; j < Pa; ++j)
-                        out[boff + row_start + (j - i - 1)] = internal[boff + colU_off(j) + i];
+                        out[boff + row_start + (j - i - 1)] = internal[ub + colU_off(j, Pa) + i];
                 }
             }
         }
+    });
+}
+
+int hs_l1gs_set_variant(hs_l1gs_t h, int variant) {
+    return guard([&] {
+        HS_REQUIRE(h, HS_EINVAL, "NULL preconditioner handle");
+        HS_REQUIRE(variant >= 0 && variant <= 2, HS_EINVAL, "unknown kernel variant");
+        h->variant = variant;
     });
 }
 

@@ -75,10 +75,12 @@ _sig("hs_l1gs_apply", [vp, vp, vp, C.c_int, vp, C.c_int])
 _sig("hs_l1gs_info", [vp, vp])
 _sig("hs_l1gs_get", [vp, C.c_int, vp])
 _sig("hs_l1gs_destroy", [vp])
+_sig("hs_l1gs_set_variant", [vp, C.c_int])
 
 EXPORTED = [
     "hs_init", "hs_last_error", "hs_last_error_aux", "hs_device_bytes",
     "hs_l1gs_build", "hs_l1gs_apply", "hs_l1gs_info", "hs_l1gs_get", "hs_l1gs_destroy",
+    "hs_l1gs_set_variant",
     "hs_csr_create", "hs_csr_spmv", "hs_csr_info", "hs_csr_get", "hs_csr_destroy",
     "hs_cheb_build", "hs_cheb_smooth", "hs_cheb_apply", "hs_cheb_info", "hs_cheb_get",
     "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve",

@@ -323,6 +323,10 @@ class L1GSPreconditioner:
     def handle(self):
         return self._h
 
+    def set_variant(self, variant: int):
+        """Selects the kernel variant of the partsize-64 symmetric sweep (0-4)."""
+        L.check(L.lib.hs_l1gs_set_variant(self._h, int(variant)))
+
     def _get(self, what):
         size = self.n if what == L.GET_D_DL1 else self.packed_size
         out = np.zeros(size)

@@ -94,6 +94,11 @@ int hs_l1gs_apply(hs_l1gs_t h, double *y, const double *x, int loc, void *stream
 int hs_l1gs_info(hs_l1gs_t h, int64_t *out);
 /* Copies D_DL1 (n) or SL/SU (packed_size) in the reference layout to host `out`. */
 int hs_l1gs_get(hs_l1gs_t h, int what, double *out);
+/* Kernel variant of the partsize-64 symmetric sweep (results are identical):
+ * 0 column loads straight into registers; 1 and 2 chunks streamed through an
+ * LDS ring of 8 / 16 chunks, 12 / 24 steps ahead (2 is the default).  Used by
+ * the autotuner.  No counterpart in the reference (one kernel per sweep). */
+int hs_l1gs_set_variant(hs_l1gs_t h, int variant);
 int hs_l1gs_destroy(hs_l1gs_t h);
 
 /* ---- device sparse operator (CSR) ------------------------------------- */

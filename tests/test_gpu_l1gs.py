@@ -202,3 +202,28 @@ def test_c2_poisson128_parity(gpu):
     assert set(np.unique(D)) == {6.0, 8.5}
     assert int((D == 8.5).sum()) == 2 * 126 * 126
     np.testing.assert_array_equal(y, yo)  # one in-partition neighbour per side: exact
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16"])
+def test_p64_symmetric_kernel_variants(gpu, precision):
+    """Every kernel variant of the partsize-64 symmetric sweep gives the same bits."""
+    import oracle as O
+    hs = gpu
+    n = 64 * 37 + 19  # ragged last partition takes the generic path
+    A = _random_sym(n, 0.004, 11)
+    x = np.random.default_rng(3).uniform(-1, 1, n)
+    ref = O.l1gs_build(A, 64, is_sym=True, sweep="symmetric", gpu=(256, 64))
+    yo = O.l1gs_ldiv(ref, x)
+    P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=64)(A, 64, isSymA=True,
+                                                                  precision=precision)
+    ys = []
+    for v in (0, 1, 2):
+        P.set_variant(v)
+        ys.append(hs.backslash(P, x))
+    np.testing.assert_array_equal(ys[0], ys[1])
+    np.testing.assert_array_equal(ys[0], ys[2])
+    tol = {"fp64": TOL, "fp32": 2e-6, "bf16": 2e-2}[precision]
+    assert relerr(ys[0], yo) < tol
+    if precision == "fp64":
+        np.testing.assert_array_equal(P.sweep.lop.L.SLbuffer, ref.SL)
+        np.testing.assert_array_equal(P.sweep.uop.U.SUbuffer, ref.SU)
