@@ -54,6 +54,19 @@ def time_kernel(fn, steps, warmup, stream):
     return start.elapsed_time(end) / steps  # ms per launch
 
 
+def pcg_bench(hs, M, b, P, label, reps=3):
+    """Time `reps` PCG solves (x0 = 0, rtol 1e-8) after one warm-up solve."""
+    sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8)
+    times = []
+    for _ in range(reps):
+        sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8)
+        times.append(sol.seconds)
+    t = float(np.median(times))
+    return {"name": label, "iters": sol.iters, "status": sol.status,
+            "true_relres": sol.true_relres, "ms_per_solve": round(t * 1e3, 3),
+            "solves_per_s": round(1.0 / t, 3), "ms_per_iter": round(t * 1e3 / max(sol.iters, 1), 4)}
+
+
 def cpu_baseline(A, partsize, budget_s=12.0):
     """The oracle's packed symmetric sweep (KA.CPU mapping, chunks = host cores)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -189,6 +202,25 @@ def main():
         "kernel_variant_ms": kernel_variants,
         "kernel_variant": int(best),
     }
+    if not args.quick:
+        log("[bench] PCG on 128^3 (rtol 1e-8)")
+        M = hs.DeviceMatrix(A)
+        bvec = torch.tensor(pb.uniform_rhs(n, 0), device="cuda")
+        pcg = [pcg_bench(hs, M, bvec, prec, "L1GS P=64 symmetric, FP64 smoother")]
+        for pname in ("fp32", "bf16"):
+            pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname)
+            pv.set_variant(int(best))
+            pcg.append(pcg_bench(hs, M, bvec, pv, f"L1GS P=64 symmetric, {pname.upper()} smoother"))
+            del pv
+        c4 = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend()), A)
+        pcg.append(pcg_bench(hs, M, bvec, c4, "Chebyshev 4th kind degree 3, FP64"))
+        c4f = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend(),
+                                                    precision="fp32"), A)
+        pcg.append(pcg_bench(hs, M, bvec, c4f, "Chebyshev 4th kind degree 3, FP32"))
+        base = pcg[0]["solves_per_s"]
+        for r in pcg:
+            r["speedup_vs_fp64_l1gs"] = round(r["solves_per_s"] / base, 3)
+        out["pcg_128"] = pcg
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(A, P)

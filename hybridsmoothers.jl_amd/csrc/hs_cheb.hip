@@ -44,7 +44,6 @@ __global__ void k_l1inv(int64_t n, const int64_t *__restrict__ rp, const double 
 struct PowState {
     double rho;
     double sumsq;
-    unsigned counter;
     int stopped;
 };
 
@@ -55,7 +54,7 @@ __global__ __launch_bounds__(256) void k_pow_spmv(int64_t n, const int64_t *__re
                                                   const double *__restrict__ l1inv,
                                                   const double *__restrict__ x,
                                                   double *__restrict__ w, double *partials,
-                                                  PowState *st) {
+                                                  const PowState *st) {
     __shared__ double sm[4];
     if (st->stopped) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,9 +66,18 @@ __global__ __launch_bounds__(256) void k_pow_spmv(int64_t n, const int64_t *__re
         sq = wi * wi;
     }
     double bs = block_sum(sq, sm);
-    double total;
-    if (grid_sum(bs, partials, &st->counter, &st->sumsq, sm, &total) && threadIdx.x == 0) {
-        double rho = sqrt(total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// rho = norm(w) from the block partials (one block)
+__global__ __launch_bounds__(256) void k_pow_fin(const double *__restrict__ partials,
+                                                 unsigned count, PowState *st) {
+    __shared__ double sm[4];
+    if (st->stopped) return;
+    double tot = finalize_sum(partials, count, 1, sm);
+    if (threadIdx.x == 0) {
+        double rho = sqrt(tot);
+        st->sumsq = tot;
         st->rho = rho;
         if (rho == 0.0) st->stopped = 1;  // iszero(rho) && break
     }
@@ -382,7 +390,7 @@ int hs_cheb_build(int64_t n, const int64_t *colptr, const int64_t *rowval, const
         {
             DBuf<double> v(n), w(n), partials(g);
             DBuf<PowState> ps(1);
-            PowState init{1.0, 0.0, 0u, 0};
+            PowState init{1.0, 0.0, 0};
             HS_CHECK_HIP(hipMemcpyAsync(ps.get(), &init, sizeof init, hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, st, n, v.get(), 1.0 / std::sqrt((double)n));
             check_launch();
@@ -390,6 +398,7 @@ int hs_cheb_build(int64_t n, const int64_t *colptr, const int64_t *rowval, const
                 hipLaunchKernelGGL(k_pow_spmv, dim3(g), dim3(256), 0, st, n, s->A.rowptr.get(),
                                    s->A.col.get(), s->A.val.get(), s->l1inv.get(), v.get(), w.get(),
                                    partials.get(), ps.get());
+                hipLaunchKernelGGL(k_pow_fin, dim3(1), dim3(256), 0, st, partials.get(), g, ps.get());
                 hipLaunchKernelGGL(k_pow_scale, dim3(g), dim3(256), 0, st, n, w.get(), v.get(),
                                    ps.get());
             }

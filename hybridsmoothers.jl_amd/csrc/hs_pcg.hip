@@ -26,7 +26,6 @@ struct CGDev {
     double res2, b2;
     long long iter, itmax;
     int done, status;
-    unsigned cnt[4];
 };
 
 __global__ void k_cg_start(int64_t n, const double *__restrict__ b, double *__restrict__ x,
@@ -52,8 +51,16 @@ __global__ __launch_bounds__(256) void k_cg_dot0(int64_t n, const double *__rest
         v = r[i] * zi;
     }
     double bs = block_sum(v, sm);
-    double tot;
-    if (grid_sum(bs, partials, &st->cnt[0], &st->gamma, sm, &tot) && threadIdx.x == 0) {
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// gamma = r'z, rnorm0, eps and the trivial-convergence test (one block)
+__global__ __launch_bounds__(256) void k_cg_fin0(const double *__restrict__ partials,
+                                                 unsigned count, CGDev *st) {
+    __shared__ double sm[4];
+    double tot = finalize_sum(partials, count, 1, sm);
+    if (threadIdx.x == 0) {
+        st->gamma = tot;
         double rn = sqrt(tot);
         st->rnorm0 = rn;
         st->rnorm = rn;
@@ -85,8 +92,17 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, const int64_t *__res
         d = p[i] * qi;
     }
     double bs = block_sum(d, sm);
-    double tot;
-    if (grid_sum(bs, partials, &st->cnt[1], &st->delta, sm, &tot) && threadIdx.x == 0) {
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// delta = p'q (one block)
+__global__ __launch_bounds__(256) void k_cg_fin_delta(const double *__restrict__ partials,
+                                                      unsigned count, CGDev *st) {
+    __shared__ double sm[4];
+    if (st->done) return;
+    double tot = finalize_sum(partials, count, 1, sm);
+    if (threadIdx.x == 0) {
+        st->delta = tot;
         if (!(tot > 0.0)) {  // breakdown: A (or M) not positive definite
             st->done = 1;
             st->status = 2;
@@ -115,8 +131,17 @@ __global__ __launch_bounds__(256) void k_cg_dot(int64_t n, const double *__restr
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double v = (i < n) ? r[i] * z[i] : 0.0;
     double bs = block_sum(v, sm);
-    double tot;
-    if (grid_sum(bs, partials, &st->cnt[2], &st->gnext, sm, &tot) && threadIdx.x == 0) {
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// gamma' = r'z, stopping test, beta (one block)
+__global__ __launch_bounds__(256) void k_cg_fin_gamma(const double *__restrict__ partials,
+                                                      unsigned count, CGDev *st) {
+    __shared__ double sm[4];
+    if (st->done) return;
+    double tot = finalize_sum(partials, count, 1, sm);
+    if (threadIdx.x == 0) {
+        st->gnext = tot;
         double rn = sqrt(tot);
         st->rnorm = rn;
         st->iter += 1;
@@ -159,7 +184,6 @@ __global__ __launch_bounds__(256) void k_cg_resid(int64_t n, const int64_t *__re
                                                   const double *__restrict__ b,
                                                   double *partials, CGDev *st) {
     __shared__ double sm[4];
-    __shared__ unsigned s_ticket;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double t2 = 0.0, b2 = 0.0;
     if (i < n) {
@@ -173,23 +197,17 @@ __global__ __launch_bounds__(256) void k_cg_resid(int64_t n, const int64_t *__re
     if (threadIdx.x == 0) {
         partials[2 * blockIdx.x] = s1;
         partials[2 * blockIdx.x + 1] = s2;
-        __threadfence();
-        s_ticket = atomicAdd(&st->cnt[3], 1u);
     }
-    __syncthreads();
-    if (s_ticket != gridDim.x - 1) return;
-    __threadfence();
-    double a = 0.0, c = 0.0;
-    for (unsigned blk = threadIdx.x; blk < gridDim.x; blk += blockDim.x) {
-        a += partials[2 * blk];
-        c += partials[2 * blk + 1];
-    }
-    a = block_sum(a, sm);
-    c = block_sum(c, sm);
+}
+
+__global__ __launch_bounds__(256) void k_cg_fin_resid(const double *__restrict__ partials,
+                                                      unsigned count, CGDev *st) {
+    __shared__ double sm[4];
+    double a = finalize_sum(partials, count, 2, sm);
+    double c = finalize_sum(partials + 1, count, 2, sm);
     if (threadIdx.x == 0) {
         st->res2 = a;
         st->b2 = c;
-        st->cnt[3] = 0u;
     }
 }
 
@@ -268,12 +286,14 @@ static void enqueue_iteration(const CSR &A, const PrecRef &M, CGWork &w, hipStre
     const int *done = &d->done;
     hipLaunchKernelGGL(k_cg_spmv, dim3(g), dim3(256), 0, st, n, A.rowptr.get(), A.col.get(),
                        A.val.get(), w.p.get(), w.q.get(), w.partials.get(), d);
+    hipLaunchKernelGGL(k_cg_fin_delta, dim3(1), dim3(256), 0, st, w.partials.get(), g, d);
     hipLaunchKernelGGL(k_cg_axpy, dim3(g), dim3(256), 0, st, n, w.p.get(), w.q.get(), w.x.get(),
                        w.r.get(), d);
     check_launch();
     apply_prec(M, w.r.get(), w.z.get(), n, st, done, d);
     hipLaunchKernelGGL(k_cg_dot, dim3(g), dim3(256), 0, st, n, w.r.get(), w.z.get(),
                        w.partials.get(), d);
+    hipLaunchKernelGGL(k_cg_fin_gamma, dim3(1), dim3(256), 0, st, w.partials.get(), g, d);
     hipLaunchKernelGGL(k_cg_pupd, dim3(g), dim3(256), 0, st, n, w.z.get(), w.p.get(), d);
     check_launch();
 }
@@ -303,6 +323,7 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
     apply_prec(M, w.r.get(), w.z.get(), n, st, nullptr, nullptr);
     hipLaunchKernelGGL(k_cg_dot0, dim3(g), dim3(256), 0, st, n, w.r.get(), w.z.get(), w.p.get(),
                        w.partials.get(), w.st.get());
+    hipLaunchKernelGGL(k_cg_fin0, dim3(1), dim3(256), 0, st, w.partials.get(), g, w.st.get());
     check_launch();
 
     if (block < 1) block = 1;
@@ -334,6 +355,7 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
     // true residual ||b - A x|| / ||b|| in FP64
     hipLaunchKernelGGL(k_cg_resid, dim3(g), dim3(256), 0, st, n, A.rowptr.get(), A.col.get(),
                        A.val.get(), w.x.get(), bvec, w.partials.get(), w.st.get());
+    hipLaunchKernelGGL(k_cg_fin_resid, dim3(1), dim3(256), 0, st, w.partials.get(), g, w.st.get());
     check_launch();
     HS_CHECK_HIP(hipMemcpyAsync(xvec, w.x.get(), n * 8, hipMemcpyDeviceToDevice, st));
     HS_CHECK_HIP(hipMemcpyAsync(w.host, w.st.get(), sizeof(CGDev), hipMemcpyDeviceToHost, st));

@@ -87,31 +87,16 @@ __device__ __forceinline__ double block_sum(double v, double *smem4) {
     return s;
 }
 
-// Grid-wide deterministic sum: every block writes its partial, the last block
-// to finish (ticket from an atomic counter) adds the partials in index order
-// and writes *out; it also resets the counter for the next launch.  Returns
-// true in the threads of the block that produced the result; thread 0 of that
-// block also returns the sum in *total.
-__device__ __forceinline__ bool grid_sum(double blockval, double *partials, unsigned *counter,
-                                         double *out, double *smem4, double *total) {
-    __shared__ unsigned s_ticket;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = blockval;
-        __threadfence();
-        s_ticket = atomicAdd(counter, 1u);
-    }
-    __syncthreads();
-    if (s_ticket != gridDim.x - 1) return false;
-    __threadfence();
+// Grid-wide deterministic sums are done in two launches: every block stores
+// its partial (plain store), then one block adds the partials in index order
+// (finalize_sum).  Kernel boundaries give the visibility; a "last block
+// finishes" scheme would need an agent-scope release in every block, which on
+// this chip writes back the XCD's L2 each time.
+__device__ __forceinline__ double finalize_sum(const double *__restrict__ partials, unsigned count,
+                                              unsigned stride, double *smem4) {
     double v = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) v += partials[b];
-    double s = block_sum(v, smem4);
-    if (threadIdx.x == 0) {
-        *out = s;
-        *counter = 0u;
-        *total = s;
-    }
-    return true;
+    for (unsigned b = threadIdx.x; b < count; b += blockDim.x) v += partials[(size_t)b * stride];
+    return block_sum(v, smem4);  // valid in thread 0
 }
 
 }  // namespace hs
