@@ -30,12 +30,11 @@ enum : int {
 
 // ---- setup ----------------------------------------------------------------
 // l1inv_i = 1 / sum_j |a_ij| in row order (_l1_rowsum, :27-36, then 1 ./ l1)
-__global__ void k_l1inv(int64_t n, const int64_t *__restrict__ rp, const double *__restrict__ v,
-                        double *__restrict__ l1inv, float *__restrict__ l1inv32) {
+__global__ void k_l1inv(SellView<double> A, double *__restrict__ l1inv,
+                        float *__restrict__ l1inv32) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0;
-    for (int64_t p = rp[i]; p < rp[i + 1]; ++p) s += fabs(v[p]);
+    if (i >= A.n) return;
+    double s = row_abs_sum(A, i);
     double r = 1.0 / s;
     l1inv[i] = r;
     if (l1inv32) l1inv32[i] = (float)r;
@@ -48,9 +47,7 @@ struct PowState {
 };
 
 // w = (A v) .* l1inv and its squared norm (power iteration, :40-54)
-__global__ __launch_bounds__(256) void k_pow_spmv(int64_t n, const int64_t *__restrict__ rp,
-                                                  const int32_t *__restrict__ ci,
-                                                  const double *__restrict__ v,
+__global__ __launch_bounds__(256) void k_pow_spmv(int64_t n, SellView<double> A,
                                                   const double *__restrict__ l1inv,
                                                   const double *__restrict__ x,
                                                   double *__restrict__ w, double *partials,
@@ -60,7 +57,7 @@ __global__ __launch_bounds__(256) void k_pow_spmv(int64_t n, const int64_t *__re
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double sq = 0.0;
     if (i < n) {
-        double wi = row_dot(rp, ci, v, x, i);
+        double wi = row_dot(A, x, i);
         wi = wi * l1inv[i];
         w[i] = wi;
         sq = wi * wi;
@@ -98,8 +95,7 @@ __global__ void k_fill(int64_t n, double *__restrict__ v, double a) {
 // ---- 4th kind -------------------------------------------------------------
 // init: r0 = b - A x (or b when x0 = 0) and z1 = a1*0 + b1*l1inv*r0
 template <class T>
-__global__ void k_c4_init(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                          const T *__restrict__ v, const double *__restrict__ x,
+__global__ void k_c4_init(int64_t n, SellView<T> A, const double *__restrict__ x,
                           const double *__restrict__ b, const T *__restrict__ l1inv,
                           T *__restrict__ r, T *__restrict__ z1, double *__restrict__ xout, T a1,
                           T b1, int flags, const int *__restrict__ done) {
@@ -107,7 +103,7 @@ __global__ void k_c4_init(int64_t n, const int64_t *__restrict__ rp, const int32
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     T ri = (T)b[i];
-    if (!(flags & F_X_ZERO)) ri = ri - row_dot(rp, ci, v, x, i);
+    if (!(flags & F_X_ZERO)) ri = ri - row_dot(A, x, i);
     if (flags & F_WRITE_R) r[i] = ri;
     T zi = a1 * (T)0 + b1 * l1inv[i] * ri;
     if (flags & F_HAS_NEXT) z1[i] = zi;
@@ -116,15 +112,14 @@ __global__ void k_c4_init(int64_t n, const int64_t *__restrict__ rp, const int32
 
 // step k: r_k = r_{k-1} - A z_k; x += z_k; z_{k+1} = an*z_k + bn*l1inv*r_k
 template <class T, class TR>
-__global__ void k_c4_step(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                          const T *__restrict__ v, const T *__restrict__ zk,
+__global__ void k_c4_step(int64_t n, SellView<T> A, const T *__restrict__ zk,
                           const TR *__restrict__ rin, T *__restrict__ rout,
                           const T *__restrict__ l1inv, double *__restrict__ x,
                           T *__restrict__ znext, T an, T bn, int flags, const int *__restrict__ done) {
     if (done && *done) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    T ri = row_sub(rp, ci, v, zk, i, (T)rin[i]);
+    T ri = row_sub(A, zk, i, (T)rin[i]);
     if (flags & F_WRITE_R) rout[i] = ri;
     T zi = zk[i];
     double xi = (flags & F_X_ZERO) ? 0.0 : x[i];
@@ -140,15 +135,14 @@ __global__ void k_c4_step(int64_t n, const int64_t *__restrict__ rp, const int32
 // ---- 1st kind -------------------------------------------------------------
 // init: r0 = l1inv*(b - A x)*inv_rho; z0 = c*r0
 template <class T>
-__global__ void k_c1_init(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                          const T *__restrict__ v, const double *__restrict__ x,
+__global__ void k_c1_init(int64_t n, SellView<T> A, const double *__restrict__ x,
                           const double *__restrict__ b, const T *__restrict__ l1inv,
                           T *__restrict__ r, T *__restrict__ z0, double *__restrict__ xout,
                           T inv_rho, T c, int flags, const int *__restrict__ done) {
     if (done && *done) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    T tmp = (flags & F_X_ZERO) ? (T)0 : row_dot(rp, ci, v, x, i);
+    T tmp = (flags & F_X_ZERO) ? (T)0 : row_dot(A, x, i);
     T ri = l1inv[i] * ((T)b[i] - tmp) * inv_rho;
     r[i] = ri;
     T zi = c * ri;
@@ -158,8 +152,7 @@ __global__ void k_c1_init(int64_t n, const int64_t *__restrict__ rp, const int32
 
 // step k < d: x += z_{k-1}; r -= l1inv*(A z_{k-1})*inv_rho; z_k = c1 z_{k-1} + c2 r
 template <class T>
-__global__ void k_c1_step(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                          const T *__restrict__ v, const T *__restrict__ zp, T *__restrict__ r,
+__global__ void k_c1_step(int64_t n, SellView<T> A, const T *__restrict__ zp, T *__restrict__ r,
                           const T *__restrict__ l1inv, double *__restrict__ x,
                           T *__restrict__ znext, T inv_rho, T c1, T c2, int flags,
                           const int *__restrict__ done) {
@@ -169,7 +162,7 @@ __global__ void k_c1_step(int64_t n, const int64_t *__restrict__ rp, const int32
     T zi = zp[i];
     double xi = (flags & F_X_ZERO) ? 0.0 : x[i];
     xi = xi + (double)zi;
-    T tmp = row_dot(rp, ci, v, zp, i);
+    T tmp = row_dot(A, zp, i);
     T ri = r[i] - l1inv[i] * tmp * inv_rho;
     r[i] = ri;
     T zn = c1 * zi + c2 * ri;
@@ -230,17 +223,10 @@ static void cheb_run(ChebPlan &s, double *x, const double *b, bool as_prec, hipS
                      const int *done) {
     const int64_t n = s.n;
     const unsigned g = grid_for(n, 256);
-    const int64_t *rp = s.A.rowptr.get();
-    const int32_t *ci = s.A.col.get();
-    const T *v;
+    const SellView<T> Av = s.A.template view_as<T>();
     const T *l1;
-    if constexpr (sizeof(T) == 8) {
-        v = s.A.val.get();
-        l1 = s.l1inv.get();
-    } else {
-        v = s.A.val32.get();
-        l1 = s.l1inv32.get();
-    }
+    if constexpr (sizeof(T) == 8) l1 = s.l1inv.get();
+    else l1 = s.l1inv32.get();
     T *r = s.ptr<T>(s.r);
     T *zb[2] = {s.ptr<T>(s.za), s.ptr<T>(s.zb)};
     const int d = s.degree;
@@ -252,7 +238,7 @@ static void cheb_run(ChebPlan &s, double *x, const double *b, bool as_prec, hipS
         };
         if (d == 0) {
             // r = b - A x; z = 0; x unchanged
-            hipLaunchKernelGGL((k_c4_init<T>), dim3(g), dim3(256), 0, st, n, rp, ci, v, x, b, l1, r,
+            hipLaunchKernelGGL((k_c4_init<T>), dim3(g), dim3(256), 0, st, n, Av, x, b, l1, r,
                                zb[0], x, (T)0, (T)0, xz | F_WRITE_R, done);
             hipLaunchKernelGGL((k_zero<T>), dim3(g), dim3(256), 0, st, n, zb[0], done);
             if (as_prec) hipLaunchKernelGGL((k_zero<double>), dim3(g), dim3(256), 0, st, n, x, done);
@@ -265,7 +251,7 @@ static void cheb_run(ChebPlan &s, double *x, const double *b, bool as_prec, hipS
         // In preconditioner mode r0 = b is read straight from b by step 1.
         int fl = xz | F_HAS_NEXT | (as_prec ? 0 : F_WRITE_R);
         if (as_prec && d == 1) fl |= F_ADD_NEXT;
-        hipLaunchKernelGGL((k_c4_init<T>), dim3(g), dim3(256), 0, st, n, rp, ci, v, x, b, l1, r,
+        hipLaunchKernelGGL((k_c4_init<T>), dim3(g), dim3(256), 0, st, n, Av, x, b, l1, r,
                            zb[0], x, a1, b1, fl, done);
         check_launch();
         int cur = 0;
@@ -283,10 +269,10 @@ static void cheb_run(ChebPlan &s, double *x, const double *b, bool as_prec, hipS
                 if (as_prec && k + 1 == d) f |= F_ADD_NEXT;
             }
             if (as_prec && k == 1) {
-                hipLaunchKernelGGL((k_c4_step<T, double>), dim3(g), dim3(256), 0, st, n, rp, ci, v,
+                hipLaunchKernelGGL((k_c4_step<T, double>), dim3(g), dim3(256), 0, st, n, Av,
                                    zb[cur], b, r, l1, x, zb[cur ^ 1], an, bn, f, done);
             } else {
-                hipLaunchKernelGGL((k_c4_step<T, T>), dim3(g), dim3(256), 0, st, n, rp, ci, v,
+                hipLaunchKernelGGL((k_c4_step<T, T>), dim3(g), dim3(256), 0, st, n, Av,
                                    zb[cur], r, r, l1, x, zb[cur ^ 1], an, bn, f, done);
             }
             check_launch();
@@ -300,7 +286,7 @@ static void cheb_run(ChebPlan &s, double *x, const double *b, bool as_prec, hipS
     const T c = (T)(2.0 / (1.0 + s.a));
     int fl = xz;
     if (as_prec && d == 1) fl |= F_ADD_NEXT;
-    hipLaunchKernelGGL((k_c1_init<T>), dim3(g), dim3(256), 0, st, n, rp, ci, v, x, b, l1, r,
+    hipLaunchKernelGGL((k_c1_init<T>), dim3(g), dim3(256), 0, st, n, Av, x, b, l1, r,
                        zb[0], x, inv_rho, c, fl, done);
     check_launch();
     int cur = 0;
@@ -326,7 +312,7 @@ static void cheb_run(ChebPlan &s, double *x, const double *b, bool as_prec, hipS
         T c2 = (T)(4.0 * rsn / (1.0 - ad));
         int f = (k == 1) ? xz : 0;
         if (k + 1 == d) f |= F_ADD_NEXT;  // the final x += z needs no SpMV
-        hipLaunchKernelGGL((k_c1_step<T>), dim3(g), dim3(256), 0, st, n, rp, ci, v, zb[cur], r, l1,
+        hipLaunchKernelGGL((k_c1_step<T>), dim3(g), dim3(256), 0, st, n, Av, zb[cur], r, l1,
                            x, zb[cur ^ 1], inv_rho, c1, c2, f, done);
         check_launch();
         cur ^= 1;
@@ -383,8 +369,8 @@ int hs_cheb_build(int64_t n, const int64_t *colptr, const int64_t *rowval, const
             s->A.ensure_fp32(st);
         }
         const unsigned g = grid_for(n, 256);
-        hipLaunchKernelGGL(k_l1inv, dim3(g), dim3(256), 0, st, n, s->A.rowptr.get(),
-                           s->A.val.get(), s->l1inv.get(), s->l1inv32.get());
+        hipLaunchKernelGGL(k_l1inv, dim3(g), dim3(256), 0, st, s->A.view(), s->l1inv.get(),
+                           s->l1inv32.get());
         check_launch();
         // power iteration for rho(M^-1 A) in FP64 (:40-54)
         {
@@ -395,9 +381,8 @@ int hs_cheb_build(int64_t n, const int64_t *colptr, const int64_t *rowval, const
             hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, st, n, v.get(), 1.0 / std::sqrt((double)n));
             check_launch();
             for (int it = 0; it < power_iters; ++it) {
-                hipLaunchKernelGGL(k_pow_spmv, dim3(g), dim3(256), 0, st, n, s->A.rowptr.get(),
-                                   s->A.col.get(), s->A.val.get(), s->l1inv.get(), v.get(), w.get(),
-                                   partials.get(), ps.get());
+                hipLaunchKernelGGL(k_pow_spmv, dim3(g), dim3(256), 0, st, n, s->A.view(),
+                                   s->l1inv.get(), v.get(), w.get(), partials.get(), ps.get());
                 hipLaunchKernelGGL(k_pow_fin, dim3(1), dim3(256), 0, st, partials.get(), g, ps.get());
                 hipLaunchKernelGGL(k_pow_scale, dim3(g), dim3(256), 0, st, n, w.get(), v.get(),
                                    ps.get());

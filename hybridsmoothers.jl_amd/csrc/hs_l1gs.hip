@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <memory>
 #include <type_traits>
 #include <vector>
@@ -597,11 +598,23 @@ __device__ __forceinline__ TC sweep_ring64(TC xr, TC dr, TC rr, int lane,
     return (lane == 0) ? q : yfin;
 }
 
-template <class TS, class TC, int NR, int RUN>
+// Row prologue of the fused CG form: r -= alpha q; x += alpha p (the k_cg_axpy
+// arithmetic); returns the updated r, the sweep's input.
+__device__ __forceinline__ double cg_prologue(const CGFuse &f, int64_t i) {
+    const double alpha = f.st->gamma / f.st->delta;
+    const double *p = (f.st->iter & 1) ? f.pb : f.pa;
+    const double ri = f.r[i] - alpha * f.q[i];
+    f.r[i] = ri;
+    f.x[i] = f.x[i] + alpha * p[i];
+    return ri;
+}
+
+template <class TS, class TC, int NR, int RUN, bool CG>
 __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
                                                      const TS *__restrict__ T,
                                                      const TC *__restrict__ D, int64_t n,
-                                                     int64_t nparts, const int *__restrict__ done) {
+                                                     int64_t nparts, const int *__restrict__ done,
+                                                     CGFuse f) {
     if (done && *done) return;
     __shared__ TS ring[NR * 64];
     const int lane = threadIdx.x;
@@ -613,7 +626,9 @@ __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
     int64_t lb, ub;
     block_bases(k, 64, Pa, MODE_SYM, lb, ub);
     const bool live = lane < Pa;
-    const TC xr = live ? (TC)x[s + lane] : (TC)0;
+    double xin = 0.0;
+    if (live) xin = CG ? cg_prologue(f, s + lane) : x[s + lane];
+    const TC xr = (TC)xin;
     const TC dr = live ? D[s + lane] : (TC)1;
     const TC rr = (TC)1 / dr;
     TC yr;
@@ -625,21 +640,20 @@ __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
         yr = sweep_generic<64, MODE_SYM, true, TS, TC>(xr, dr, rr, lane, Pa, live, T + lb, T + ub);
     }
     if (live) y[s + lane] = (double)yr;
+    if constexpr (CG) {
+        double v = live ? xin * (double)yr : 0.0;  // r'z partial of this partition
+        v = wave_sum(v);
+        if (lane == 0) f.partials[k] = v;
+    }
 }
 
-// One partition per PM-lane segment, 256 threads per workgroup.
-// TS: storage type of the triangles, TC: arithmetic type (double or float).
-// x and y may alias (each partition reads its rows before writing them).
-template <int PM, int MODE, class TS, class TC>
-__global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
-                                                   const TS *__restrict__ T,
-                                                   const TC *__restrict__ D, int64_t n, int P,
-                                                   int64_t nparts, const int *__restrict__ done) {
-    if (done && *done) return;  // converged CG: speculative launches do nothing
-    constexpr int SEGS = 256 / PM;
-    const int lane = threadIdx.x % PM;
-    const int64_t k = (int64_t)blockIdx.x * SEGS + threadIdx.x / PM;
-    if (k >= nparts) return;
+// One partition of the segment kernel; returns r_i * z_i of the lane's row
+// (fused CG form) or 0.
+template <int PM, int MODE, class TS, class TC, bool CG>
+__device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
+                                                 const TS *__restrict__ T,
+                                                 const TC *__restrict__ D, int64_t n, int P,
+                                                 int64_t k, int lane, const CGFuse &f) {
     const int64_t s = k * (int64_t)P;
     const int64_t rem = n - s;
     const int Pa = (int)(rem < P ? rem : P);
@@ -649,7 +663,9 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
     const TS *Lk = T + lb;
     const TS *Uk = T + ub;
 
-    const TC xr = live ? (TC)x[s + lane] : (TC)0;
+    double xin = 0.0;
+    if (live) xin = CG ? cg_prologue(f, s + lane) : x[s + lane];
+    const TC xr = (TC)xin;
     const TC dr = live ? D[s + lane] : (TC)1;
     const TC rr = (TC)1 / dr;
     TC yr;
@@ -661,6 +677,33 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
         yr = sweep_generic<PM, MODE, true, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
     }
     if (live) y[s + lane] = (double)yr;
+    return live ? xin * (double)yr : 0.0;
+}
+
+// One partition per PM-lane segment, 256 threads per workgroup.
+// TS: storage type of the triangles, TC: arithmetic type (double or float).
+// x and y may alias (each partition reads its rows before writing them).
+template <int PM, int MODE, class TS, class TC, bool CG = false>
+__global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
+                                                   const TS *__restrict__ T,
+                                                   const TC *__restrict__ D, int64_t n, int P,
+                                                   int64_t nparts, const int *__restrict__ done,
+                                                   CGFuse f = CGFuse{}) {
+    if (done && *done) return;  // converged CG: speculative launches do nothing
+    constexpr int SEGS = 256 / PM;
+    const int lane = threadIdx.x % PM;
+    const int64_t k = (int64_t)blockIdx.x * SEGS + threadIdx.x / PM;
+    if constexpr (CG) {
+        // whole blocks stay for the block-wide r'z partial
+        __shared__ double sm[4];
+        double v = 0.0;
+        if (k < nparts) v = sweep_seg_body<PM, MODE, TS, TC, true>(x, y, T, D, n, P, k, lane, f);
+        v = block_sum(v, sm);
+        if (threadIdx.x == 0) f.partials[blockIdx.x] = v;
+        return;
+    }
+    if (k >= nparts) return;
+    (void)sweep_seg_body<PM, MODE, TS, TC, false>(x, y, T, D, n, P, k, lane, f);
 }
 
 // Partitions larger than a wave: one thread per partition, rows in sequence,
@@ -821,11 +864,11 @@ static void launch_packed(const L1GSPlan &h, const double *x, double *y, const T
         const TS *T = reinterpret_cast<const TS *>(h.T.get());
         const dim3 grid(grid_for(h.nparts, 1)), block(64);
         if (h.variant == 1)
-            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 8, 12>), grid, block, 0, st, x, y, T, D,
-                               h.n, h.nparts, done);
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 8, 12, false>), grid, block, 0, st, x, y, T,
+                               D, h.n, h.nparts, done, CGFuse{});
         else
-            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 16, 24>), grid, block, 0, st, x, y, T, D,
-                               h.n, h.nparts, done);
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 16, 24, false>), grid, block, 0, st, x, y, T,
+                               D, h.n, h.nparts, done, CGFuse{});
         return;
     }
     if (h.P <= 4) launch_seg<4, TS, TC>(h, x, y, D, st, done);
@@ -846,6 +889,53 @@ static void launch_packed(const L1GSPlan &h, const double *x, double *y, const T
             hipLaunchKernelGGL((k_sweep_thread<MODE_SYM, TS, TC>), dim3(grid), dim3(64), 0, st, x,
                                y, T, D, h.n, h.P, h.nparts, done);
     }
+}
+
+template <int PM, class TS, class TC>
+static unsigned launch_seg_fused(const L1GSPlan &h, double *z, const TC *D, hipStream_t st,
+                                 const int *done, const CGFuse &f) {
+    constexpr int SEGS = 256 / PM;
+    const unsigned grid = grid_for(h.nparts, SEGS);
+    hipLaunchKernelGGL((k_sweep_seg<PM, MODE_SYM, TS, TC, true>), dim3(grid), dim3(256), 0, st,
+                       (const double *)nullptr, z, reinterpret_cast<const TS *>(h.T.get()), D, h.n,
+                       (int)h.P, h.nparts, done, f);
+    return grid;
+}
+
+template <class TS, class TC>
+static unsigned launch_fused(const L1GSPlan &h, double *z, const TC *D, hipStream_t st,
+                             const CGFuse &f) {
+    const int *done = reinterpret_cast<const int *>(reinterpret_cast<const char *>(f.st) +
+                                                    offsetof(CGDev, done));
+    if (h.P == 64 && h.variant > 0) {
+        const TS *T = reinterpret_cast<const TS *>(h.T.get());
+        const dim3 grid(grid_for(h.nparts, 1)), block(64);
+        if (h.variant == 1)
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 8, 12, true>), grid, block, 0, st,
+                               (const double *)nullptr, z, T, D, h.n, h.nparts, done, f);
+        else
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 16, 24, true>), grid, block, 0, st,
+                               (const double *)nullptr, z, T, D, h.n, h.nparts, done, f);
+        return (unsigned)h.nparts;
+    }
+    if (h.P <= 4) return launch_seg_fused<4, TS, TC>(h, z, D, st, done, f);
+    if (h.P <= 8) return launch_seg_fused<8, TS, TC>(h, z, D, st, done, f);
+    if (h.P <= 16) return launch_seg_fused<16, TS, TC>(h, z, D, st, done, f);
+    if (h.P <= 32) return launch_seg_fused<32, TS, TC>(h, z, D, st, done, f);
+    return launch_seg_fused<64, TS, TC>(h, z, D, st, done, f);
+}
+
+bool l1gs_can_fuse(const L1GSPlan &h) {
+    return h.cache == HS_CACHE_PACKED && h.sweep == HS_SWEEP_SYMMETRIC && h.P <= 64 && h.n > 0;
+}
+
+unsigned l1gs_apply_fused(const L1GSPlan &h, double *z, hipStream_t st, const CGFuse &f) {
+    unsigned count;
+    if (h.prec == HS_PREC_FP64) count = launch_fused<double, double>(h, z, h.D.get(), st, f);
+    else if (h.prec == HS_PREC_FP32) count = launch_fused<float, float>(h, z, h.Df.get(), st, f);
+    else count = launch_fused<__hip_bfloat16, float>(h, z, h.Df.get(), st, f);
+    check_launch();
+    return count;
 }
 
 // Device-side application y = M^-1 x (x, y device pointers; may alias).

@@ -14,19 +14,12 @@
 // Dot products are deterministic (fixed block tree, fixed final order).
 #include "hs_plans.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <memory>
 
 namespace hs {
-
-struct CGDev {
-    double gamma, delta, gnext, beta;
-    double rnorm, rnorm0, eps, atol, rtol;
-    double res2, b2;
-    long long iter, itmax;
-    int done, status;
-};
 
 __global__ void k_cg_start(int64_t n, const double *__restrict__ b, double *__restrict__ x,
                            double *__restrict__ r) {
@@ -55,9 +48,9 @@ __global__ __launch_bounds__(256) void k_cg_dot0(int64_t n, const double *__rest
 }
 
 // gamma = r'z, rnorm0, eps and the trivial-convergence test (one block)
-__global__ __launch_bounds__(256) void k_cg_fin0(const double *__restrict__ partials,
-                                                 unsigned count, CGDev *st) {
-    __shared__ double sm[4];
+__global__ __launch_bounds__(FIN_BLOCK) void k_cg_fin0(const double *__restrict__ partials,
+                                                       unsigned count, CGDev *st) {
+    __shared__ double sm[16];
     double tot = finalize_sum(partials, count, 1, sm);
     if (threadIdx.x == 0) {
         st->gamma = tot;
@@ -76,9 +69,7 @@ __global__ __launch_bounds__(256) void k_cg_fin0(const double *__restrict__ part
 }
 
 // q = A p; delta = p'q
-__global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, const int64_t *__restrict__ rp,
-                                                 const int32_t *__restrict__ ci,
-                                                 const double *__restrict__ v,
+__global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, SellView<double> A,
                                                  const double *__restrict__ p,
                                                  double *__restrict__ q, double *partials,
                                                  CGDev *st) {
@@ -87,7 +78,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, const int64_t *__res
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.0;
     if (i < n) {
-        double qi = row_dot(rp, ci, v, p, i);
+        double qi = row_dot(A, p, i);
         q[i] = qi;
         d = p[i] * qi;
     }
@@ -96,9 +87,9 @@ __global__ __launch_bounds__(256) void k_cg_spmv(int64_t n, const int64_t *__res
 }
 
 // delta = p'q (one block)
-__global__ __launch_bounds__(256) void k_cg_fin_delta(const double *__restrict__ partials,
-                                                      unsigned count, CGDev *st) {
-    __shared__ double sm[4];
+__global__ __launch_bounds__(FIN_BLOCK) void k_cg_fin_delta(const double *__restrict__ partials,
+                                                            unsigned count, CGDev *st) {
+    __shared__ double sm[16];
     if (st->done) return;
     double tot = finalize_sum(partials, count, 1, sm);
     if (threadIdx.x == 0) {
@@ -108,6 +99,44 @@ __global__ __launch_bounds__(256) void k_cg_fin_delta(const double *__restrict__
             st->status = 2;
         }
     }
+}
+
+// Fused CG form with an L1GS preconditioner, iteration k (st->iter == k):
+// p_k = z + beta p_{k-1} (p_0 = z) formed on the fly, q = A p_k, p_k'q
+// partials.  p_k goes to pbuf[k & 1] while p_{k-1} is gathered from
+// pbuf[(k + 1) & 1], so no thread reads a value another thread rewrites.
+__global__ __launch_bounds__(256) void k_cg_spmv_fused(int64_t n, SellView<double> A,
+                                                       const double *__restrict__ z,
+                                                       double *__restrict__ pa,
+                                                       double *__restrict__ pb,
+                                                       double *__restrict__ q, double *partials,
+                                                       const CGDev *st) {
+    __shared__ double sm[4];
+    if (st->done) return;
+    const long long k = st->iter;
+    const bool first = (k == 0);
+    const double beta = first ? 0.0 : st->beta;
+    const double *pold = (k & 1) ? pa : pb;
+    double *pnew = (k & 1) ? pb : pa;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.0;
+    if (i < n) {
+        const int64_t s = i >> 6;
+        const int l = (int)(i & 63);
+        const int64_t e = A.sptr[s + 1];
+        double acc = 0.0;
+        for (int64_t p = A.sptr[s] + l; p < e; p += 64) {
+            const int32_t c = A.col[p];
+            const double pc = first ? z[c] : z[c] + beta * pold[c];
+            acc = acc + A.val[p] * pc;
+        }
+        const double pi = first ? z[i] : z[i] + beta * pold[i];
+        pnew[i] = pi;
+        q[i] = acc;
+        d = pi * acc;
+    }
+    double bs = block_sum(d, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
 // x += alpha p; r -= alpha q
@@ -135,9 +164,9 @@ __global__ __launch_bounds__(256) void k_cg_dot(int64_t n, const double *__restr
 }
 
 // gamma' = r'z, stopping test, beta (one block)
-__global__ __launch_bounds__(256) void k_cg_fin_gamma(const double *__restrict__ partials,
-                                                      unsigned count, CGDev *st) {
-    __shared__ double sm[4];
+__global__ __launch_bounds__(FIN_BLOCK) void k_cg_fin_gamma(const double *__restrict__ partials,
+                                                            unsigned count, CGDev *st) {
+    __shared__ double sm[16];
     if (st->done) return;
     double tot = finalize_sum(partials, count, 1, sm);
     if (threadIdx.x == 0) {
@@ -177,9 +206,7 @@ __global__ void k_copy(int64_t n, const double *__restrict__ a, double *__restri
 }
 
 // ||b - A x||^2 and ||b||^2
-__global__ __launch_bounds__(256) void k_cg_resid(int64_t n, const int64_t *__restrict__ rp,
-                                                  const int32_t *__restrict__ ci,
-                                                  const double *__restrict__ v,
+__global__ __launch_bounds__(256) void k_cg_resid(int64_t n, SellView<double> A,
                                                   const double *__restrict__ x,
                                                   const double *__restrict__ b,
                                                   double *partials, CGDev *st) {
@@ -188,7 +215,7 @@ __global__ __launch_bounds__(256) void k_cg_resid(int64_t n, const int64_t *__re
     double t2 = 0.0, b2 = 0.0;
     if (i < n) {
         double bi = b[i];
-        double t = bi - row_dot(rp, ci, v, x, i);
+        double t = bi - row_dot(A, x, i);
         t2 = t * t;
         b2 = bi * bi;
     }
@@ -200,9 +227,9 @@ __global__ __launch_bounds__(256) void k_cg_resid(int64_t n, const int64_t *__re
     }
 }
 
-__global__ __launch_bounds__(256) void k_cg_fin_resid(const double *__restrict__ partials,
-                                                      unsigned count, CGDev *st) {
-    __shared__ double sm[4];
+__global__ __launch_bounds__(FIN_BLOCK) void k_cg_fin_resid(const double *__restrict__ partials,
+                                                            unsigned count, CGDev *st) {
+    __shared__ double sm[16];
     double a = finalize_sum(partials, count, 2, sm);
     double c = finalize_sum(partials + 1, count, 2, sm);
     if (threadIdx.x == 0) {
@@ -216,7 +243,7 @@ __global__ __launch_bounds__(256) void k_cg_fin_resid(const double *__restrict__
 // ---------------------------------------------------------------------------
 struct CGWork {
     int64_t n = 0;
-    DBuf<double> x, r, z, p, q, b, partials;
+    DBuf<double> x, r, z, p, p2, q, b, partials;
     DBuf<CGDev> st;
     CGDev *host = nullptr;  // pinned mirror of the scalars
     hipStream_t stream = nullptr;
@@ -235,8 +262,10 @@ struct CGWork {
         }
         if (nn != n) {
             n = nn;
-            x.alloc(n); r.alloc(n); z.alloc(n); p.alloc(n); q.alloc(n); b.alloc(n);
-            partials.alloc(2 * (size_t)grid_for(n, 256));
+            x.alloc(n); r.alloc(n); z.alloc(n); p.alloc(n); p2.alloc(n); q.alloc(n); b.alloc(n);
+            // block partials of the 256-row kernels (two values each for the
+            // residual) or one per partition of the fused sweep (>= 1 row each)
+            partials.alloc(std::max<size_t>(2 * (size_t)grid_for(n, 256), (size_t)n));
             drop_graph();
         }
     }
@@ -279,21 +308,40 @@ static void apply_prec(const PrecRef &M, const double *r, double *z, int64_t n, 
     }
 }
 
+// One iteration of the fused form: 4 launches, see k_cg_spmv_fused and CGFuse.
+static void enqueue_iteration_fused(const CSR &A, const PrecRef &M, CGWork &w, hipStream_t st) {
+    const int64_t n = w.n;
+    const unsigned g = grid_for(n, 256);
+    CGDev *d = w.st.get();
+    const L1GSPlan &plan = *reinterpret_cast<const L1GSPlan *>(M.plan);
+    hipLaunchKernelGGL(k_cg_spmv_fused, dim3(g), dim3(256), 0, st, n, A.view(), w.z.get(),
+                       w.p.get(), w.p2.get(), w.q.get(), w.partials.get(), d);
+    hipLaunchKernelGGL(k_cg_fin_delta, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), g, d);
+    check_launch();
+    // the sweep reads p_k from the buffer the SpMV above wrote; a graph
+    // replays fixed pointers, so both buffers are passed and the parity of
+    // st->iter picks one on the device
+    CGFuse f{w.r.get(), w.q.get(), w.p.get(), w.p2.get(), w.x.get(), w.partials.get(), d};
+    unsigned cnt = l1gs_apply_fused(plan, w.z.get(), st, f);
+    hipLaunchKernelGGL(k_cg_fin_gamma, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), cnt, d);
+    check_launch();
+}
+
 static void enqueue_iteration(const CSR &A, const PrecRef &M, CGWork &w, hipStream_t st) {
     const int64_t n = w.n;
     const unsigned g = grid_for(n, 256);
     CGDev *d = w.st.get();
     const int *done = &d->done;
-    hipLaunchKernelGGL(k_cg_spmv, dim3(g), dim3(256), 0, st, n, A.rowptr.get(), A.col.get(),
-                       A.val.get(), w.p.get(), w.q.get(), w.partials.get(), d);
-    hipLaunchKernelGGL(k_cg_fin_delta, dim3(1), dim3(256), 0, st, w.partials.get(), g, d);
+    hipLaunchKernelGGL(k_cg_spmv, dim3(g), dim3(256), 0, st, n, A.view(), w.p.get(), w.q.get(),
+                       w.partials.get(), d);
+    hipLaunchKernelGGL(k_cg_fin_delta, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), g, d);
     hipLaunchKernelGGL(k_cg_axpy, dim3(g), dim3(256), 0, st, n, w.p.get(), w.q.get(), w.x.get(),
                        w.r.get(), d);
     check_launch();
     apply_prec(M, w.r.get(), w.z.get(), n, st, done, d);
     hipLaunchKernelGGL(k_cg_dot, dim3(g), dim3(256), 0, st, n, w.r.get(), w.z.get(),
                        w.partials.get(), d);
-    hipLaunchKernelGGL(k_cg_fin_gamma, dim3(1), dim3(256), 0, st, w.partials.get(), g, d);
+    hipLaunchKernelGGL(k_cg_fin_gamma, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), g, d);
     hipLaunchKernelGGL(k_cg_pupd, dim3(g), dim3(256), 0, st, n, w.z.get(), w.p.get(), d);
     check_launch();
 }
@@ -323,10 +371,16 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
     apply_prec(M, w.r.get(), w.z.get(), n, st, nullptr, nullptr);
     hipLaunchKernelGGL(k_cg_dot0, dim3(g), dim3(256), 0, st, n, w.r.get(), w.z.get(), w.p.get(),
                        w.partials.get(), w.st.get());
-    hipLaunchKernelGGL(k_cg_fin0, dim3(1), dim3(256), 0, st, w.partials.get(), g, w.st.get());
+    hipLaunchKernelGGL(k_cg_fin0, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), g, w.st.get());
     check_launch();
 
     if (block < 1) block = 1;
+    const bool fused = M.kind == HS_PCG_L1GS &&
+                       l1gs_can_fuse(*reinterpret_cast<const L1GSPlan *>(M.plan));
+    auto iteration = [&](hipStream_t s) {
+        if (fused) enqueue_iteration_fused(A, M, w, s);
+        else enqueue_iteration(A, M, w, s);
+    };
     if (use_graph) {
         const uint64_t key[3] = {A.serial, M.serial(), (uint64_t)M.kind};
         if (!w.graph || w.graph_block != block || key[0] != w.graph_key[0] ||
@@ -334,7 +388,7 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
             w.drop_graph();
             hipGraph_t graph;
             HS_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < block; ++k) enqueue_iteration(A, M, w, st);
+            for (int k = 0; k < block; ++k) iteration(st);
             HS_CHECK_HIP(hipStreamEndCapture(st, &graph));
             HS_CHECK_HIP(hipGraphInstantiate(&w.graph, graph, nullptr, nullptr, 0));
             (void)hipGraphDestroy(graph);
@@ -349,13 +403,13 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
         if (use_graph) {
             HS_CHECK_HIP(hipGraphLaunch(w.graph, st));
         } else {
-            for (int k = 0; k < block; ++k) enqueue_iteration(A, M, w, st);
+            for (int k = 0; k < block; ++k) iteration(st);
         }
     }
     // true residual ||b - A x|| / ||b|| in FP64
-    hipLaunchKernelGGL(k_cg_resid, dim3(g), dim3(256), 0, st, n, A.rowptr.get(), A.col.get(),
-                       A.val.get(), w.x.get(), bvec, w.partials.get(), w.st.get());
-    hipLaunchKernelGGL(k_cg_fin_resid, dim3(1), dim3(256), 0, st, w.partials.get(), g, w.st.get());
+    hipLaunchKernelGGL(k_cg_resid, dim3(g), dim3(256), 0, st, n, A.view(), w.x.get(), bvec,
+                       w.partials.get(), w.st.get());
+    hipLaunchKernelGGL(k_cg_fin_resid, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), g, w.st.get());
     check_launch();
     HS_CHECK_HIP(hipMemcpyAsync(xvec, w.x.get(), n * 8, hipMemcpyDeviceToDevice, st));
     HS_CHECK_HIP(hipMemcpyAsync(w.host, w.st.get(), sizeof(CGDev), hipMemcpyDeviceToHost, st));

@@ -2,6 +2,7 @@
 // src/chebyshev.jl:144 and the system matrix of the CG driver) and its SpMV.
 #include "hs_sparse.hpp"
 
+#include <algorithm>
 #include <memory>
 
 namespace hs {
@@ -24,14 +25,6 @@ void csc_to_csr_host(int64_t n, const int64_t *colptr, const int64_t *rowval, co
         }
 }
 
-__global__ void k_csr_convert(int64_t n, int64_t nnz, const int64_t *__restrict__ rp1,
-                              const int64_t *__restrict__ ci1, int64_t *__restrict__ rp,
-                              int32_t *__restrict__ ci) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t <= n) rp[t] = rp1[t] - 1;
-    if (t < nnz) ci[t] = (int32_t)(ci1[t] - 1);
-}
-
 template <class TD>
 __global__ void k_to_type(int64_t n, const double *__restrict__ s, TD *__restrict__ d) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -44,42 +37,69 @@ void csr_upload(CSR &A, int64_t n, const int64_t *rp1, const int64_t *ci1, const
     const int64_t nnz = rp1[n] - 1;
     A.n = n;
     A.nnz = nnz;
-    A.rowptr.alloc(n + 1);
-    A.col.alloc(nnz);
-    A.val.alloc(nnz);
-    DBuf<int64_t> trp(n + 1), tci(nnz);
-    HS_CHECK_HIP(hipMemcpyAsync(trp.get(), rp1, (n + 1) * 8, hipMemcpyHostToDevice, st));
-    if (nnz) {
-        HS_CHECK_HIP(hipMemcpyAsync(tci.get(), ci1, nnz * 8, hipMemcpyHostToDevice, st));
-        HS_CHECK_HIP(hipMemcpyAsync(A.val.get(), cv, nnz * 8, hipMemcpyHostToDevice, st));
+    A.nslices = (n + 63) / 64;
+    A.h_rp.resize(n + 1);
+    A.h_ci.resize(nnz);
+    A.h_cv.assign(cv, cv + nnz);
+    for (int64_t i = 0; i <= n; ++i) A.h_rp[i] = rp1[i] - 1;
+    for (int64_t p = 0; p < nnz; ++p) A.h_ci[p] = (int32_t)(ci1[p] - 1);
+    // slice widths and offsets
+    std::vector<int64_t> sp(A.nslices + 1, 0);
+    for (int64_t s = 0; s < A.nslices; ++s) {
+        int64_t w = 0;
+        for (int64_t i = 64 * s; i < std::min<int64_t>(64 * s + 64, n); ++i)
+            w = std::max<int64_t>(w, A.h_rp[i + 1] - A.h_rp[i]);
+        sp[s + 1] = sp[s] + 64 * w;
     }
-    int64_t work = (nnz > n + 1) ? nnz : n + 1;
-    hipLaunchKernelGGL(k_csr_convert, dim3(grid_for(work, 256)), dim3(256), 0, st, n, nnz,
-                       trp.get(), tci.get(), A.rowptr.get(), A.col.get());
-    check_launch();
-    HS_CHECK_HIP(hipStreamSynchronize(st));
+    A.nent = sp[A.nslices];
+    std::vector<int32_t> col(A.nent);
+    std::vector<double> val(A.nent);
+    for (int64_t s = 0; s < A.nslices; ++s) {
+        const int64_t w = (sp[s + 1] - sp[s]) / 64;
+        for (int l = 0; l < 64; ++l) {
+            const int64_t i = 64 * s + l;
+            const int64_t len = i < n ? A.h_rp[i + 1] - A.h_rp[i] : 0;
+            for (int64_t k = 0; k < w; ++k) {
+                const int64_t q = sp[s] + 64 * k + l;
+                if (k < len) {
+                    col[q] = A.h_ci[A.h_rp[i] + k];
+                    val[q] = A.h_cv[A.h_rp[i] + k];
+                } else {  // padding: the row's own column, value zero
+                    col[q] = (int32_t)(i < n ? i : n - 1);
+                    val[q] = 0.0;
+                }
+            }
+        }
+    }
+    A.sptr.alloc(A.nslices + 1);
+    A.col.alloc(A.nent);
+    A.val.alloc(A.nent);
+    HS_CHECK_HIP(hipMemcpyAsync(A.sptr.get(), sp.data(), (A.nslices + 1) * 8,
+                                hipMemcpyHostToDevice, st));
+    if (A.nent) {
+        HS_CHECK_HIP(hipMemcpyAsync(A.col.get(), col.data(), A.nent * 4, hipMemcpyHostToDevice, st));
+        HS_CHECK_HIP(hipMemcpyAsync(A.val.get(), val.data(), A.nent * 8, hipMemcpyHostToDevice, st));
+    }
+    HS_CHECK_HIP(hipStreamSynchronize(st));  // the host staging vectors go out of scope
 }
 
 void CSR::ensure_fp32(hipStream_t st) {
-    if (val32.get() || nnz == 0) return;
-    val32.alloc(nnz);
-    hipLaunchKernelGGL((k_to_type<float>), dim3(grid_for(nnz, 256)), dim3(256), 0, st, nnz,
+    if (val32.get() || nent == 0) return;
+    val32.alloc(nent);
+    hipLaunchKernelGGL((k_to_type<float>), dim3(grid_for(nent, 256)), dim3(256), 0, st, nent,
                        val.get(), val32.get());
     check_launch();
 }
 
 // y = A x, one thread per row, sequential row sums (mul! of SparseMatricesCSR)
-__global__ void k_spmv(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                       const double *__restrict__ v, const double *__restrict__ x,
-                       double *__restrict__ y) {
+__global__ void k_spmv(SellView<double> A, const double *__restrict__ x, double *__restrict__ y) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = row_dot(rp, ci, v, x, i);
+    if (i < A.n) y[i] = row_dot(A, x, i);
 }
 
 void spmv_device(const CSR &A, const double *x, double *y, hipStream_t st) {
     if (A.n == 0) return;
-    hipLaunchKernelGGL(k_spmv, dim3(grid_for(A.n, 256)), dim3(256), 0, st, A.n, A.rowptr.get(),
-                       A.col.get(), A.val.get(), x, y);
+    hipLaunchKernelGGL(k_spmv, dim3(grid_for(A.n, 256)), dim3(256), 0, st, A.view(), x, y);
     check_launch();
 }
 
@@ -144,16 +164,15 @@ int hs_csr_info(hs_csr_t A, int64_t *out) {
 int hs_csr_get(hs_csr_t A, int64_t *rowptr, int64_t *colval, double *nzval) {
     return guard([&] {
         HS_REQUIRE(A && rowptr && (A->nnz == 0 || (colval && nzval)), HS_EINVAL, "NULL argument");
-        std::vector<int64_t> rp(A->n + 1);
-        std::vector<int32_t> ci(A->nnz);
-        if (A->n) HS_CHECK_HIP(hipMemcpy(rp.data(), A->rowptr.get(), (A->n + 1) * 8, hipMemcpyDeviceToHost));
-        else rp[0] = 0;
-        if (A->nnz) {
-            HS_CHECK_HIP(hipMemcpy(ci.data(), A->col.get(), A->nnz * 4, hipMemcpyDeviceToHost));
-            HS_CHECK_HIP(hipMemcpy(nzval, A->val.get(), A->nnz * 8, hipMemcpyDeviceToHost));
+        if (A->n == 0) {
+            rowptr[0] = 1;
+            return;
         }
-        for (int64_t i = 0; i <= A->n; ++i) rowptr[i] = rp[i] + 1;
-        for (int64_t p = 0; p < A->nnz; ++p) colval[p] = (int64_t)ci[p] + 1;
+        for (int64_t i = 0; i <= A->n; ++i) rowptr[i] = A->h_rp[i] + 1;
+        for (int64_t p = 0; p < A->nnz; ++p) {
+            colval[p] = (int64_t)A->h_ci[p] + 1;
+            nzval[p] = A->h_cv[p];
+        }
     });
 }
 

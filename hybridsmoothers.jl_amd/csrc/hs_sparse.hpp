@@ -9,19 +9,35 @@
 namespace hs {
 
 // ---------------------------------------------------------------------------
-// CSR on the device: 0-based int32 column indices, int64 row pointers.
-// Values in FP64 (always) and optionally FP32 (reduced-precision smoothers).
+// The device operator: sliced ELL with 64-row slices.  Slice s holds rows
+// [64s, 64s+64); its entries are stored column-major -- entry k of row 64s+l
+// at sptr[s] + 64k + l -- padded to the longest row of the slice with
+// (column = the row itself, value = 0).  One wave reads one slice with fully
+// coalesced loads, and every row is still summed in its CSR order (padding
+// comes last and adds zero), so results equal a row-by-row CSR product.
+// A host copy of the CSR (0-based) is kept for getters.
 // ---------------------------------------------------------------------------
 struct CGWork;                 // CG workspace (hs_pcg.hip), owned by the matrix
 void cgwork_free(CGWork *w);
 
+template <class T>
+struct SellView {
+    const int64_t *sptr;
+    const int32_t *col;
+    const T *val;
+    int64_t n;
+};
+
 struct CSR {
     uint64_t serial = next_serial();
-    int64_t n = 0, nnz = 0;
-    DBuf<int64_t> rowptr;
+    int64_t n = 0, nnz = 0, nslices = 0, nent = 0;
+    DBuf<int64_t> sptr;
     DBuf<int32_t> col;
     DBuf<double> val;
     DBuf<float> val32;
+    std::vector<int64_t> h_rp;  // host CSR, 0-based
+    std::vector<int32_t> h_ci;
+    std::vector<double> h_cv;
     CGWork *cg = nullptr;
 
     CSR() = default;
@@ -31,6 +47,14 @@ struct CSR {
         if (cg) cgwork_free(cg);
     }
     void ensure_fp32(hipStream_t st);
+    SellView<double> view() const { return {sptr.get(), col.get(), val.get(), n}; }
+    SellView<float> view32() const { return {sptr.get(), col.get(), val32.get(), n}; }
+    template <class T> SellView<T> view_as() const {
+        if constexpr (sizeof(T) == 8) return view();
+        else return view32();
+    }
+    // algorithmic bytes of one product: values, column indices, slice offsets
+    int64_t bytes(int vbytes) const { return nent * (vbytes + 4) + (nslices + 1) * 8; }
 };
 
 // Host CSR (1-based Int64, as SparseMatrixCSR stores it) from 1-based CSC by a
@@ -38,30 +62,43 @@ struct CSR {
 void csc_to_csr_host(int64_t n, const int64_t *colptr, const int64_t *rowval, const double *nzval,
                      std::vector<int64_t> &rp, std::vector<int64_t> &ci, std::vector<double> &cv);
 
-// Uploads a 1-based CSR (host) as a device CSR.
+// Builds the device operator from a 1-based CSR (host).
 void csr_upload(CSR &A, int64_t n, const int64_t *rp1, const int64_t *ci1, const double *cv,
                 hipStream_t st);
 
 // ---------------------------------------------------------------------------
-// row dot product of a CSR row with a gathered vector, summed in row order
+// row products, summed in row order
 // ---------------------------------------------------------------------------
 template <class T, class TX>
-__device__ __forceinline__ T row_dot(const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                     const T *__restrict__ v, const TX *__restrict__ x, int64_t i) {
+__device__ __forceinline__ T row_dot(const SellView<T> &A, const TX *__restrict__ x, int64_t i) {
+    const int64_t s = i >> 6;
+    const int l = (int)(i & 63);
+    const int64_t e = A.sptr[s + 1];
     T acc = (T)0;
-    const int64_t p1 = rp[i + 1];
-    for (int64_t p = rp[i]; p < p1; ++p) acc = acc + v[p] * (T)x[ci[p]];
+    for (int64_t p = A.sptr[s] + l; p < e; p += 64) acc = acc + A.val[p] * (T)x[A.col[p]];
     return acc;
 }
 
 // r_i - a_1 x_1 - a_2 x_2 - ... term by term (5-argument mul! with alpha = -1, beta = 1)
 template <class T, class TX>
-__device__ __forceinline__ T row_sub(const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                     const T *__restrict__ v, const TX *__restrict__ x, int64_t i,
+__device__ __forceinline__ T row_sub(const SellView<T> &A, const TX *__restrict__ x, int64_t i,
                                      T r) {
-    const int64_t p1 = rp[i + 1];
-    for (int64_t p = rp[i]; p < p1; ++p) r = r - v[p] * (T)x[ci[p]];
+    const int64_t s = i >> 6;
+    const int l = (int)(i & 63);
+    const int64_t e = A.sptr[s + 1];
+    for (int64_t p = A.sptr[s] + l; p < e; p += 64) r = r - A.val[p] * (T)x[A.col[p]];
     return r;
+}
+
+// sum_j |a_ij| in row order
+template <class T>
+__device__ __forceinline__ double row_abs_sum(const SellView<T> &A, int64_t i) {
+    const int64_t s = i >> 6;
+    const int l = (int)(i & 63);
+    const int64_t e = A.sptr[s + 1];
+    double acc = 0.0;
+    for (int64_t p = A.sptr[s] + l; p < e; p += 64) acc += fabs((double)A.val[p]);
+    return acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -92,11 +129,27 @@ __device__ __forceinline__ double block_sum(double v, double *smem4) {
 // (finalize_sum).  Kernel boundaries give the visibility; a "last block
 // finishes" scheme would need an agent-scope release in every block, which on
 // this chip writes back the XCD's L2 each time.
+// Sum over a block of up to 1024 threads (blockDim a multiple of 64); the
+// result is valid in thread 0.
+__device__ __forceinline__ double block_sum_wide(double v, double *smem16) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) smem16[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nw; ++k) s += smem16[k];
+    __syncthreads();
+    return s;
+}
+
+constexpr int FIN_BLOCK = 1024;  // threads of the finalize kernels
+
 __device__ __forceinline__ double finalize_sum(const double *__restrict__ partials, unsigned count,
-                                              unsigned stride, double *smem4) {
+                                              unsigned stride, double *smem16) {
     double v = 0.0;
     for (unsigned b = threadIdx.x; b < count; b += blockDim.x) v += partials[(size_t)b * stride];
-    return block_sum(v, smem4);  // valid in thread 0
+    return block_sum_wide(v, smem16);  // valid in thread 0
 }
 
 }  // namespace hs
