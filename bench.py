@@ -39,6 +39,21 @@ def sweep_bytes(n, S, tri_bytes=8, vec_bytes=8, d_bytes=8):
     return 2 * S * tri_bytes + n * (vec_bytes + d_bytes + vec_bytes)
 
 
+def measured_traffic(variant):
+    """HBM bytes per launch of the headline sweep kernel, from the committed
+    FETCH_SIZE / WRITE_SIZE counter passes (scaled by a plain copy's readings),
+    or None when they do not cover the kernel variant being timed."""
+    path = os.path.join(ROOT, "profiles", "l1gs_sym_sweep_fp64_poisson128_p64_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if variant != 2:  # the counters were taken on the ring-16 / run-24 kernel
+        return None
+    return int(rec["hbm_bytes_per_dispatch"])
+
+
 def time_kernel(fn, steps, warmup, stream):
     import torch
     for _ in range(warmup):
@@ -196,7 +211,8 @@ def main():
                                f"one application per step",
                    "rows": n, "nnz": int(A.nnz), "partsize": P, "packed_entries": S},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(int(best)) if g == 128 and P == 64 else None,
                      "algorithmic_bytes": nbytes},
         "sweeps": variants,
         "kernel_variant_ms": kernel_variants,
