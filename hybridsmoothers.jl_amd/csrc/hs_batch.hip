@@ -1,0 +1,889 @@
+// hs_batch.hip -- batched persistent PCG: many SPD systems of uneven size
+// solved together, each with an L1GS symmetric preconditioner, as a serving
+// engine (submit / step / collect).
+//
+// No counterpart in the reference, whose users solve one system per call
+// (solve(LinearProblem(A, b), KrylovJL_CG(); Pl = P), README.md:27-29); every
+// system here runs exactly the arithmetic of hs_pcg_solve with that
+// preconditioner (FP64 CG, Krylov.jl's stopping rule, x0 = 0).
+//
+// Device state:
+//  * one arena of HBM, carved by a host first-fit allocator into one block
+//    per resident system: its sliced-ELL matrix, row lengths, right-hand side,
+//    packed triangles, D_Dl1, CG vectors and reduction partials;
+//  * a slot table (descriptor + CG scalars per resident system);
+//  * a tile table: one entry per (system, 256-row tile) of every active system.
+//
+// One CG iteration of every active system is one "pass" of four launches:
+//   k_b_spmv      tiles: p = z + beta p, q = A p, p'q partial   (or the true
+//                 residual partials of a converged system)
+//   k_b_fin_delta systems: delta = p'q (or ||b - Ax||^2, ||b||^2)
+//   k_b_sweep     tiles: r -= alpha q, x += alpha p, z = M^-1 r, r'z partial
+//                 (a newly admitted system: r = b, x = 0, z = M^-1 b)
+//   k_b_fin_gamma systems: the CG scalars and the stopping test
+// The tile kernels are persistent: a fixed grid, each workgroup walking a
+// contiguous run of the tile table (neighbouring tiles of a system share the
+// workgroup's XCD and its L2).  A fixed number of passes is captured once in
+// a HIP graph; admission and retirement only rewrite the device tables
+// between graph launches, so the graph is never re-captured.
+//
+// Every per-system reduction sums that system's own tile partials in a fixed
+// order, so a system's result does not depend on which others share the
+// batch, and it equals hs_pcg_solve's bit for bit (same kernels' arithmetic,
+// same 256-row partials, same finalize tree).
+#include "hs_sweep.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace hs {
+namespace batch {
+
+constexpr int TILE = 256;  // rows per tile (= one 256-thread workgroup)
+
+enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_RESID = 3, PH_DONE = 4 };
+
+// Descriptor of a resident system (device array, one per slot).
+struct Sys {
+    const int64_t *sptr;
+    const int32_t *col;
+    const double *val;
+    const void *T;  // packed triangles (storage type of the engine's precision)
+    const void *D;  // D_Dl1 (FP64, or FP32 for the reduced-precision smoothers)
+    double *x, *r, *z, *pa, *pb, *q;
+    const double *b;
+    double *part;  // two partials per tile
+    int64_t n, nparts;
+    int32_t ntiles, pad;
+};
+
+struct State {
+    CGDev cg;
+    int phase, pad;
+};
+
+struct Tile {
+    int32_t slot, tile;
+};
+
+struct Counts {
+    int32_t ntiles, nactive;
+};
+
+// contiguous run of the tile table for this workgroup
+__device__ __forceinline__ void tile_range(int nt, int &t0, int &t1) {
+    const int per = (nt + (int)gridDim.x - 1) / (int)gridDim.x;
+    t0 = (int)blockIdx.x * per;
+    t1 = min(t0 + per, nt);
+}
+
+__global__ __launch_bounds__(256) void k_b_spmv(const Tile *__restrict__ tiles,
+                                                const Counts *__restrict__ cnt,
+                                                const Sys *__restrict__ sys,
+                                                const State *__restrict__ st) {
+    __shared__ double sm[4];
+    int t0, t1;
+    tile_range(cnt->ntiles, t0, t1);
+    for (int t = t0; t < t1; ++t) {
+        const Tile tl = tiles[t];
+        const int ph = st[tl.slot].phase;
+        if (ph != PH_ITER && ph != PH_RESID) continue;  // uniform in the workgroup
+        const Sys &S = sys[tl.slot];
+        const SellView<double> A{S.sptr, S.col, S.val, S.n};
+        const int64_t i = (int64_t)tl.tile * TILE + threadIdx.x;
+        if (ph == PH_ITER) {
+            const CGDev &B = st[tl.slot].cg;
+            const long long k = B.iter;
+            const bool first = (k == 0);
+            const double beta = first ? 0.0 : B.beta;
+            const double *pold = (k & 1) ? S.pa : S.pb;
+            double *pnew = (k & 1) ? S.pb : S.pa;
+            double d = 0.0;
+            if (i < S.n) d = cg_spmv_row(A, S.z, pold, pnew, S.q, first, beta, i);
+            d = block_sum(d, sm);
+            if (threadIdx.x == 0) S.part[2 * tl.tile] = d;
+        } else {
+            // true residual of a finished system, ||b - A x||^2 and ||b||^2
+            double t2 = 0.0, b2 = 0.0;
+            if (i < S.n) {
+                const double bi = S.b[i];
+                const double e = bi - row_dot(A, S.x, i);
+                t2 = e * e;
+                b2 = bi * bi;
+            }
+            t2 = block_sum(t2, sm);
+            b2 = block_sum(b2, sm);
+            if (threadIdx.x == 0) {
+                S.part[2 * tl.tile] = t2;
+                S.part[2 * tl.tile + 1] = b2;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(FIN_BLOCK) void k_b_fin_delta(const int32_t *__restrict__ act,
+                                                           const Counts *__restrict__ cnt,
+                                                           const Sys *__restrict__ sys,
+                                                           State *st) {
+    __shared__ double sm[16];
+    const int na = cnt->nactive;
+    for (int a = blockIdx.x; a < na; a += gridDim.x) {
+        const int s = act[a];
+        const int ph = st[s].phase;  // read by every thread before the barriers below
+        const Sys &S = sys[s];
+        if (ph == PH_ITER) {
+            const double tot = finalize_sum(S.part, (unsigned)S.ntiles, 2, sm);
+            if (threadIdx.x == 0 && !cg_scalars_delta(st[s].cg, tot)) st[s].phase = PH_RESID;
+        } else if (ph == PH_RESID) {
+            const double e2 = finalize_sum(S.part, (unsigned)S.ntiles, 2, sm);
+            const double b2 = finalize_sum(S.part + 1, (unsigned)S.ntiles, 2, sm);
+            if (threadIdx.x == 0) {
+                st[s].cg.res2 = e2;
+                st[s].cg.b2 = b2;
+                st[s].phase = PH_DONE;
+            }
+        }
+    }
+}
+
+// PM = partsize (16, 32 or 64): 256/PM partitions per tile, one per PM-lane segment.
+template <int PM, class TS, class TC>
+__global__ __launch_bounds__(256) void k_b_sweep(const Tile *__restrict__ tiles,
+                                                 const Counts *__restrict__ cnt,
+                                                 const Sys *__restrict__ sys,
+                                                 const State *__restrict__ st) {
+    __shared__ double sm[4];
+    constexpr int SEGS = TILE / PM;
+    int t0, t1;
+    tile_range(cnt->ntiles, t0, t1);
+    for (int t = t0; t < t1; ++t) {
+        const Tile tl = tiles[t];
+        const int ph = st[tl.slot].phase;
+        if (ph != PH_INIT && ph != PH_ITER) continue;
+        const Sys &S = sys[tl.slot];
+        const TS *T = reinterpret_cast<const TS *>(S.T);
+        const TC *D = reinterpret_cast<const TC *>(S.D);
+        const int lane = threadIdx.x % PM;
+        const int64_t k = (int64_t)tl.tile * SEGS + threadIdx.x / PM;
+        double v = 0.0;
+        if (!__all(k >= S.nparts)) {  // whole waves: see sweep_seg_body
+            if (ph == PH_INIT) {
+                // r = b, x = 0, z = M^-1 b; partial b'z = r'z
+                v = sweep_seg_body<PM, MODE_SYM, TS, TC, false>(S.b, S.z, T, D, S.n, PM, S.nparts,
+                                                               k, lane, CGFuse{});
+                const int64_t i = k * PM + lane;
+                if (k < S.nparts && i < S.n) {
+                    S.r[i] = S.b[i];
+                    S.x[i] = 0.0;
+                }
+            } else {
+                const CGFuse f{S.r, S.q, S.pa, S.pb, S.x, nullptr, &st[tl.slot].cg};
+                v = sweep_seg_body<PM, MODE_SYM, TS, TC, true>(nullptr, S.z, T, D, S.n, PM,
+                                                              S.nparts, k, lane, f);
+            }
+        }
+        v = block_sum(v, sm);
+        if (threadIdx.x == 0) S.part[2 * tl.tile] = v;
+    }
+}
+
+__global__ __launch_bounds__(FIN_BLOCK) void k_b_fin_gamma(const int32_t *__restrict__ act,
+                                                           const Counts *__restrict__ cnt,
+                                                           const Sys *__restrict__ sys,
+                                                           State *st) {
+    __shared__ double sm[16];
+    const int na = cnt->nactive;
+    for (int a = blockIdx.x; a < na; a += gridDim.x) {
+        const int s = act[a];
+        const int ph = st[s].phase;
+        if (ph != PH_INIT && ph != PH_ITER) continue;
+        const Sys &S = sys[s];
+        const double tot = finalize_sum(S.part, (unsigned)S.ntiles, 2, sm);
+        if (threadIdx.x == 0) {
+            const bool done = (ph == PH_INIT) ? cg_scalars_start(st[s].cg, tot)
+                                              : cg_scalars_next(st[s].cg, tot);
+            st[s].phase = done ? PH_RESID : PH_ITER;
+        }
+    }
+}
+
+// D_Dl1 and the packed triangles of one system from its sliced-ELL rows (the
+// rows of a symmetric matrix, summed in ascending column order exactly as
+// k_setup_rows does for a CSR or symmetric CSC).
+template <class TS, class TC>
+__global__ void k_b_setup(int64_t n, int64_t P, const int64_t *__restrict__ sptr,
+                          const int32_t *__restrict__ col, const double *__restrict__ val,
+                          const int32_t *__restrict__ rlen, double eta, TC *__restrict__ D,
+                          TS *__restrict__ T) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int64_t k = g / P;
+    const int64_t s = k * P;
+    const int64_t e = (s + P < n ? s + P : n) - 1;
+    const int64_t Pa = e - s + 1;
+    const int64_t i = g - s;
+    int64_t lb, ub;
+    block_bases(k, P, Pa, MODE_SYM, lb, ub);
+    const int64_t base = sptr[g >> 6] + (g & 63);
+    double a = 0.0, d = 0.0;
+    const int len = rlen[g];
+    for (int kk = 0; kk < len; ++kk) {
+        const int64_t p = base + 64 * (int64_t)kk;
+        const int64_t j = col[p];
+        const double v = val[p];
+        if (j == g) {
+            a = v;
+        } else if (j < s || j > e) {
+            d += fabs(v);
+        } else if (j < g) {
+            const int64_t jl = j - s;
+            T[lb + colL_off(jl, Pa) + (i - jl - 1)] = Scalar<TS>::from_f64(v);
+        } else {
+            const int64_t jl = j - s;
+            T[ub + colU_off(jl, Pa) + i] = Scalar<TS>::from_f64(v);
+        }
+    }
+    D[g] = (TC)dl1_rule(a, d, eta);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+// Placement of one system inside its arena block.  The uploaded part (matrix,
+// row lengths, right-hand side) comes first so that one copy moves it.
+struct Layout {
+    int64_t n = 0, nslices = 0, nent = 0, nparts = 0, ntiles = 0, tri = 0;
+    int64_t o_sptr = 0, o_col = 0, o_val = 0, o_rlen = 0, o_b = 0, upload = 0;
+    int64_t o_T = 0, o_D = 0, o_x = 0, o_r = 0, o_z = 0, o_pa = 0, o_pb = 0, o_q = 0, o_part = 0;
+    int64_t total = 0;
+};
+
+static Layout make_layout(int64_t n, const int64_t *ptr, int64_t P, int tsz, int dsz) {
+    Layout L;
+    L.n = n;
+    L.nslices = (n + 63) / 64;
+    for (int64_t s = 0; s < L.nslices; ++s) {
+        int64_t w = 0;
+        for (int64_t i = 64 * s; i < std::min<int64_t>(64 * s + 64, n); ++i)
+            w = std::max<int64_t>(w, ptr[i + 1] - ptr[i]);
+        L.nent += 64 * w;
+    }
+    L.nparts = (n + P - 1) / P;
+    L.ntiles = (n + TILE - 1) / TILE;
+    const int64_t Pl = n - (L.nparts - 1) * P;
+    L.tri = (L.nparts - 1) * P * (P - 1) + Pl * (Pl - 1);
+    L.o_sptr = 0;
+    L.o_col = align256(L.o_sptr + (L.nslices + 1) * 8);
+    L.o_val = align256(L.o_col + L.nent * 4);
+    L.o_rlen = align256(L.o_val + L.nent * 8);
+    L.o_b = align256(L.o_rlen + n * 4);
+    L.upload = L.o_b + n * 8;
+    L.o_T = align256(L.upload);
+    L.o_D = align256(L.o_T + L.tri * tsz);
+    L.o_x = align256(L.o_D + n * dsz);
+    L.o_r = align256(L.o_x + n * 8);
+    L.o_z = align256(L.o_r + n * 8);
+    L.o_pa = align256(L.o_z + n * 8);
+    L.o_pb = align256(L.o_pa + n * 8);
+    L.o_q = align256(L.o_pb + n * 8);
+    L.o_part = align256(L.o_q + n * 8);
+    L.total = align256(L.o_part + 2 * L.ntiles * 8);
+    return L;
+}
+
+// Sliced-ELL of the rows (1-based CSR, or the columns of a symmetric CSC),
+// row lengths and b, written into `dst` at the layout's offsets.  The slice
+// format is the device operator's (padding: the row itself, value 0).
+static void fill_upload(const Layout &L, const int64_t *ptr, const int64_t *ind,
+                        const double *val, const double *b, uint8_t *dst) {
+    const int64_t n = L.n;
+    int64_t *sp = reinterpret_cast<int64_t *>(dst + L.o_sptr);
+    int32_t *col = reinterpret_cast<int32_t *>(dst + L.o_col);
+    double *v = reinterpret_cast<double *>(dst + L.o_val);
+    int32_t *rl = reinterpret_cast<int32_t *>(dst + L.o_rlen);
+    sp[0] = 0;
+    for (int64_t s = 0; s < L.nslices; ++s) {
+        int64_t w = 0;
+        const int64_t i1 = std::min<int64_t>(64 * s + 64, n);
+        for (int64_t i = 64 * s; i < i1; ++i) w = std::max<int64_t>(w, ptr[i + 1] - ptr[i]);
+        sp[s + 1] = sp[s] + 64 * w;
+        for (int l = 0; l < 64; ++l) {
+            const int64_t i = 64 * s + l;
+            const bool row = i < n;
+            const int64_t len = row ? ptr[i + 1] - ptr[i] : 0;
+            const int64_t p0 = row ? ptr[i] - 1 : 0;
+            if (row) rl[i] = (int32_t)len;
+            for (int64_t k = 0; k < w; ++k) {
+                const int64_t q = sp[s] + 64 * k + l;
+                if (k < len) {
+                    const int64_t c = ind[p0 + k] - 1;
+                    HS_REQUIRE(c >= 0 && c < n, HS_EINVAL, "column index out of range");
+                    col[q] = (int32_t)c;
+                    v[q] = val[p0 + k];
+                } else {
+                    col[q] = (int32_t)(row ? i : n - 1);
+                    v[q] = 0.0;
+                }
+            }
+        }
+    }
+    std::memcpy(dst + L.o_b, b, n * 8);
+}
+
+// First-fit allocator of arena offsets with coalescing of free neighbours.
+class Arena {
+  public:
+    explicit Arena(int64_t cap = 0) { reset(cap); }
+    void reset(int64_t cap) {
+        cap_ = cap;
+        free_.clear();
+        if (cap > 0) free_[0] = cap;
+        used_ = 0;
+    }
+    bool alloc(int64_t size, int64_t &off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= size) {
+                off = it->first;
+                const int64_t rest = it->second - size;
+                free_.erase(it);
+                if (rest > 0) free_[off + size] = rest;
+                used_ += size;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(int64_t off, int64_t size) {
+        used_ -= size;
+        auto next = free_.lower_bound(off);
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                off = prev->first;
+                size += prev->second;
+                free_.erase(prev);
+            }
+        }
+        next = free_.lower_bound(off);
+        if (next != free_.end() && off + size == next->first) {
+            size += next->second;
+            free_.erase(next);
+        }
+        free_[off] = size;
+    }
+    int64_t used() const { return used_; }
+    int64_t capacity() const { return cap_; }
+    int64_t largest() const {
+        int64_t m = 0;
+        for (auto &kv : free_) m = std::max(m, kv.second);
+        return m;
+    }
+
+  private:
+    std::map<int64_t, int64_t> free_;
+    int64_t cap_ = 0, used_ = 0;
+};
+
+enum { SLOT_FREE = 0, SLOT_UPLOADING = 1, SLOT_ACTIVE = 2 };
+
+struct Slot {
+    int state = SLOT_FREE;
+    int64_t id = -1, off = 0;
+    Layout lay;
+    double *x_out = nullptr;
+    double rtol = 0, atol = 0;
+    int64_t itmax = 0;
+    double t_submit = 0, t_start = 0;
+    hipEvent_t ready = nullptr;
+};
+
+struct Engine {
+    hs_batch_config cfg{};
+    int tsz = 8, dsz = 8;
+    int grid = 0;
+    uint8_t *arena_dev = nullptr;
+    Arena arena;
+    uint8_t *staging = nullptr;
+    int64_t stage_off = 0;
+    hipStream_t cs = nullptr, us = nullptr;  // compute and upload streams
+    DBuf<Sys> d_sys;
+    DBuf<State> d_st;
+    DBuf<Tile> d_tiles;
+    DBuf<int32_t> d_act;
+    DBuf<Counts> d_cnt;
+    // pinned host mirrors; rewritten only after the previous step has finished
+    Sys *h_sys = nullptr;
+    State *h_st = nullptr;
+    Tile *h_tiles = nullptr;
+    int32_t *h_act = nullptr;
+    Counts *h_cnt = nullptr;
+    State *h_out = nullptr;  // scalars copied back after every step
+    int64_t max_tiles = 0;
+    std::vector<Slot> slots;
+    std::vector<int> free_slots;
+    std::vector<int> activated;  // slots whose descriptor must be uploaded
+    bool tables_dirty = false;
+    int32_t nactive = 0;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev_step = nullptr;
+    bool step_pending = false;
+    int64_t next_id = 0, steps = 0, retired = 0;
+    std::deque<hs_batch_result> results;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    ~Engine();
+    void create(const hs_batch_config &c);
+    void enqueue_pass(hipStream_t s);
+    void capture();
+    int submit(int64_t n, const int64_t *ptr, const int64_t *ind, const double *val,
+               const double *b, double *x_out, double rtol, double atol, int64_t itmax,
+               int64_t *id);
+    void wait_step();
+    void step();
+    void collect();
+};
+
+Engine::~Engine() {
+    if (cs) (void)hipStreamSynchronize(cs);
+    if (us) (void)hipStreamSynchronize(us);
+    if (graph) (void)hipGraphExecDestroy(graph);
+    for (auto &s : slots)
+        if (s.ready) (void)hipEventDestroy(s.ready);
+    if (ev_step) (void)hipEventDestroy(ev_step);
+    if (arena_dev) {
+        (void)hipFree(arena_dev);
+        g_device_bytes -= arena.capacity();
+    }
+    for (void *p : {(void *)staging, (void *)h_sys, (void *)h_st, (void *)h_tiles, (void *)h_act,
+                    (void *)h_cnt, (void *)h_out})
+        if (p) (void)hipHostFree(p);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (us) (void)hipStreamDestroy(us);
+}
+
+template <class T>
+static void host_alloc(T *&p, size_t count) {
+    HS_CHECK_HIP(hipHostMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T),
+                               hipHostMallocDefault));
+}
+
+void Engine::create(const hs_batch_config &c) {
+    cfg = c;
+    tsz = c.precision == HS_PREC_FP64 ? 8 : (c.precision == HS_PREC_FP32 ? 4 : 2);
+    dsz = c.precision == HS_PREC_FP64 ? 8 : 4;
+    int dev = 0;
+    HS_CHECK_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HS_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
+    grid = c.grid > 0 ? c.grid : 8 * prop.multiProcessorCount;
+    HS_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    HS_CHECK_HIP(hipStreamCreateWithFlags(&us, hipStreamNonBlocking));
+    HS_CHECK_HIP(hipEventCreateWithFlags(&ev_step, hipEventDisableTiming));
+    HS_CHECK_HIP(hipMalloc((void **)&arena_dev, c.arena_bytes));
+    g_device_bytes += c.arena_bytes;
+    arena.reset(c.arena_bytes);
+    host_alloc(staging, (size_t)c.staging_bytes);
+    const int64_t ms = c.max_systems;
+    // every resident row takes at least 64 arena bytes (seven FP64 vectors)
+    max_tiles = c.arena_bytes / (64 * TILE) + ms;
+    d_sys.alloc(ms);
+    d_st.alloc(ms);
+    d_tiles.alloc(max_tiles);
+    d_act.alloc(ms);
+    d_cnt.alloc(1);
+    host_alloc(h_sys, ms);
+    host_alloc(h_st, ms);
+    host_alloc(h_tiles, max_tiles);
+    host_alloc(h_act, ms);
+    host_alloc(h_cnt, 1);
+    host_alloc(h_out, ms);
+    std::memset(h_st, 0, ms * sizeof(State));
+    std::memset(h_out, 0, ms * sizeof(State));
+    std::memset(h_sys, 0, ms * sizeof(Sys));
+    h_cnt->ntiles = 0;
+    h_cnt->nactive = 0;
+    HS_CHECK_HIP(hipMemcpyAsync(d_st.get(), h_st, ms * sizeof(State), hipMemcpyHostToDevice, cs));
+    HS_CHECK_HIP(hipMemcpyAsync(d_sys.get(), h_sys, ms * sizeof(Sys), hipMemcpyHostToDevice, cs));
+    HS_CHECK_HIP(hipMemcpyAsync(d_cnt.get(), h_cnt, sizeof(Counts), hipMemcpyHostToDevice, cs));
+    slots.resize(ms);
+    for (int64_t s = ms - 1; s >= 0; --s) {
+        HS_CHECK_HIP(hipEventCreateWithFlags(&slots[s].ready, hipEventDisableTiming));
+        free_slots.push_back((int)s);
+    }
+    HS_CHECK_HIP(hipStreamSynchronize(cs));
+    capture();
+}
+
+template <int PM>
+static void launch_sweep(int prec, unsigned grid, hipStream_t s, const Tile *tiles,
+                         const Counts *cnt, const Sys *sys, const State *st) {
+    if (prec == HS_PREC_FP64)
+        hipLaunchKernelGGL((k_b_sweep<PM, double, double>), dim3(grid), dim3(TILE), 0, s, tiles,
+                           cnt, sys, st);
+    else if (prec == HS_PREC_FP32)
+        hipLaunchKernelGGL((k_b_sweep<PM, float, float>), dim3(grid), dim3(TILE), 0, s, tiles, cnt,
+                           sys, st);
+    else
+        hipLaunchKernelGGL((k_b_sweep<PM, __hip_bfloat16, float>), dim3(grid), dim3(TILE), 0, s,
+                           tiles, cnt, sys, st);
+}
+
+void Engine::enqueue_pass(hipStream_t s) {
+    const unsigned fin_grid = (unsigned)std::min<int64_t>(cfg.max_systems, 256);
+    hipLaunchKernelGGL(k_b_spmv, dim3(grid), dim3(TILE), 0, s, d_tiles.get(), d_cnt.get(),
+                       d_sys.get(), d_st.get());
+    hipLaunchKernelGGL(k_b_fin_delta, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(),
+                       d_cnt.get(), d_sys.get(), d_st.get());
+    if (cfg.partsize == 16)
+        launch_sweep<16>(cfg.precision, grid, s, d_tiles.get(), d_cnt.get(), d_sys.get(), d_st.get());
+    else if (cfg.partsize == 32)
+        launch_sweep<32>(cfg.precision, grid, s, d_tiles.get(), d_cnt.get(), d_sys.get(), d_st.get());
+    else
+        launch_sweep<64>(cfg.precision, grid, s, d_tiles.get(), d_cnt.get(), d_sys.get(), d_st.get());
+    hipLaunchKernelGGL(k_b_fin_gamma, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(),
+                       d_cnt.get(), d_sys.get(), d_st.get());
+    check_launch();
+}
+
+void Engine::capture() {
+    hipGraph_t g;
+    HS_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < cfg.passes; ++k) enqueue_pass(cs);
+    HS_CHECK_HIP(hipStreamEndCapture(cs, &g));
+    HS_CHECK_HIP(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+}
+
+int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const double *val,
+                   const double *b, double *x_out, double rtol, double atol, int64_t itmax,
+                   int64_t *id) {
+    HS_REQUIRE(n > 0 && n < (int64_t)INT32_MAX, HS_EINVAL, "system size must be in [1, 2^31-1)");
+    HS_REQUIRE(ptr[0] == 1, HS_EINVAL, "pointer array must be 1-based (ptr[0] == 1)");
+    for (int64_t i = 0; i < n; ++i)
+        HS_REQUIRE(ptr[i + 1] >= ptr[i], HS_EINVAL, "pointer array must be non-decreasing");
+    const Layout L = make_layout(n, ptr, cfg.partsize, tsz, dsz);
+    HS_REQUIRE(L.upload <= cfg.staging_bytes, HS_EINVAL,
+               "system larger than the engine's upload staging buffer");
+    HS_REQUIRE(L.total <= cfg.arena_bytes, HS_EINVAL, "system larger than the engine's arena");
+    if (free_slots.empty()) return HS_EAGAIN;
+    int64_t off = 0;
+    if (!arena.alloc(L.total, off)) return HS_EAGAIN;
+    const int s = free_slots.back();
+    try {
+        if (stage_off + L.upload > cfg.staging_bytes) {
+            HS_CHECK_HIP(hipStreamSynchronize(us));  // earlier uploads have left the staging buffer
+            stage_off = 0;
+        }
+        uint8_t *src = staging + stage_off;
+        fill_upload(L, ptr, ind, val, b, src);
+        stage_off = align256(stage_off + L.upload);
+        uint8_t *blk = arena_dev + off;
+        HS_CHECK_HIP(hipMemcpyAsync(blk, src, L.upload, hipMemcpyHostToDevice, us));
+        if (L.tri) HS_CHECK_HIP(hipMemsetAsync(blk + L.o_T, 0, L.tri * tsz, us));
+        const dim3 g(grid_for(n, 256)), t(256);
+        const auto *sp = reinterpret_cast<const int64_t *>(blk + L.o_sptr);
+        const auto *cl = reinterpret_cast<const int32_t *>(blk + L.o_col);
+        const auto *vl = reinterpret_cast<const double *>(blk + L.o_val);
+        const auto *rl = reinterpret_cast<const int32_t *>(blk + L.o_rlen);
+        if (cfg.precision == HS_PREC_FP64)
+            hipLaunchKernelGGL((k_b_setup<double, double>), g, t, 0, us, n, cfg.partsize, sp, cl,
+                               vl, rl, cfg.eta, reinterpret_cast<double *>(blk + L.o_D),
+                               reinterpret_cast<double *>(blk + L.o_T));
+        else if (cfg.precision == HS_PREC_FP32)
+            hipLaunchKernelGGL((k_b_setup<float, float>), g, t, 0, us, n, cfg.partsize, sp, cl, vl,
+                               rl, cfg.eta, reinterpret_cast<float *>(blk + L.o_D),
+                               reinterpret_cast<float *>(blk + L.o_T));
+        else
+            hipLaunchKernelGGL((k_b_setup<__hip_bfloat16, float>), g, t, 0, us, n, cfg.partsize,
+                               sp, cl, vl, rl, cfg.eta, reinterpret_cast<float *>(blk + L.o_D),
+                               reinterpret_cast<__hip_bfloat16 *>(blk + L.o_T));
+        check_launch();
+        HS_CHECK_HIP(hipEventRecord(slots[s].ready, us));
+    } catch (...) {
+        arena.release(off, L.total);
+        throw;
+    }
+    free_slots.pop_back();
+    Slot &S = slots[s];
+    S.state = SLOT_UPLOADING;
+    S.id = next_id++;
+    S.off = off;
+    S.lay = L;
+    S.x_out = x_out;
+    S.rtol = rtol;
+    S.atol = atol;
+    S.itmax = itmax;
+    S.t_submit = now();
+    *id = S.id;
+    return HS_OK;
+}
+
+void Engine::wait_step() {
+    if (!step_pending) return;
+    HS_CHECK_HIP(hipEventSynchronize(ev_step));
+    step_pending = false;
+}
+
+void Engine::step() {
+    wait_step();
+    bool any_active = false, any_uploading = false;
+    for (const Slot &S : slots) {
+        any_active |= S.state == SLOT_ACTIVE;
+        any_uploading |= S.state == SLOT_UPLOADING;
+    }
+    // nothing to iterate yet: wait for the uploads instead of returning empty-handed
+    if (!any_active && any_uploading) HS_CHECK_HIP(hipStreamSynchronize(us));
+    // admit uploaded systems
+    for (size_t s = 0; s < slots.size(); ++s) {
+        Slot &S = slots[s];
+        if (S.state != SLOT_UPLOADING) continue;
+        const hipError_t q = hipEventQuery(S.ready);
+        if (q == hipErrorNotReady) continue;
+        HS_CHECK_HIP(q);
+        S.state = SLOT_ACTIVE;
+        S.t_start = now();
+        activated.push_back((int)s);
+        tables_dirty = true;
+    }
+    if (tables_dirty) {
+        int64_t want = 0;
+        for (const Slot &S : slots)
+            if (S.state == SLOT_ACTIVE) want += S.lay.ntiles;
+        // cannot trigger: every resident row holds >= 64 arena bytes
+        HS_REQUIRE(want <= max_tiles, HS_EDEVICE, "tile table overflow");
+        int32_t nt = 0, na = 0;
+        for (size_t s = 0; s < slots.size(); ++s) {
+            const Slot &S = slots[s];
+            if (S.state != SLOT_ACTIVE) continue;
+            h_act[na++] = (int32_t)s;
+            for (int64_t t = 0; t < S.lay.ntiles; ++t) h_tiles[nt++] = Tile{(int32_t)s, (int32_t)t};
+        }
+        h_cnt->ntiles = nt;
+        h_cnt->nactive = na;
+        nactive = na;
+        for (int s : activated) {
+            const Slot &S = slots[s];
+            const Layout &L = S.lay;
+            uint8_t *blk = arena_dev + S.off;
+            Sys &d = h_sys[s];
+            d.sptr = reinterpret_cast<const int64_t *>(blk + L.o_sptr);
+            d.col = reinterpret_cast<const int32_t *>(blk + L.o_col);
+            d.val = reinterpret_cast<const double *>(blk + L.o_val);
+            d.T = blk + L.o_T;
+            d.D = blk + L.o_D;
+            d.x = reinterpret_cast<double *>(blk + L.o_x);
+            d.r = reinterpret_cast<double *>(blk + L.o_r);
+            d.z = reinterpret_cast<double *>(blk + L.o_z);
+            d.pa = reinterpret_cast<double *>(blk + L.o_pa);
+            d.pb = reinterpret_cast<double *>(blk + L.o_pb);
+            d.q = reinterpret_cast<double *>(blk + L.o_q);
+            d.b = reinterpret_cast<const double *>(blk + L.o_b);
+            d.part = reinterpret_cast<double *>(blk + L.o_part);
+            d.n = L.n;
+            d.nparts = L.nparts;
+            d.ntiles = (int32_t)L.ntiles;
+            State &h = h_st[s];
+            std::memset(&h, 0, sizeof(State));
+            h.cg.atol = S.atol;
+            h.cg.rtol = S.rtol;
+            h.cg.itmax = S.itmax;
+            h.phase = PH_INIT;
+            HS_CHECK_HIP(hipStreamWaitEvent(cs, S.ready, 0));
+            HS_CHECK_HIP(hipMemcpyAsync(d_sys.get() + s, &h_sys[s], sizeof(Sys),
+                                        hipMemcpyHostToDevice, cs));
+            HS_CHECK_HIP(hipMemcpyAsync(d_st.get() + s, &h_st[s], sizeof(State),
+                                        hipMemcpyHostToDevice, cs));
+        }
+        activated.clear();
+        if (nt) HS_CHECK_HIP(hipMemcpyAsync(d_tiles.get(), h_tiles, nt * sizeof(Tile),
+                                            hipMemcpyHostToDevice, cs));
+        if (na) HS_CHECK_HIP(hipMemcpyAsync(d_act.get(), h_act, na * sizeof(int32_t),
+                                            hipMemcpyHostToDevice, cs));
+        HS_CHECK_HIP(hipMemcpyAsync(d_cnt.get(), h_cnt, sizeof(Counts), hipMemcpyHostToDevice, cs));
+        tables_dirty = false;
+    }
+    if (nactive == 0) return;
+    HS_CHECK_HIP(hipGraphLaunch(graph, cs));
+    HS_CHECK_HIP(hipMemcpyAsync(h_out, d_st.get(), slots.size() * sizeof(State),
+                                hipMemcpyDeviceToHost, cs));
+    HS_CHECK_HIP(hipEventRecord(ev_step, cs));
+    step_pending = true;
+    ++steps;
+}
+
+void Engine::collect() {
+    if (!step_pending) return;
+    wait_step();
+    std::vector<int> fin;
+    for (size_t s = 0; s < slots.size(); ++s)
+        if (slots[s].state == SLOT_ACTIVE && h_out[s].phase == PH_DONE) fin.push_back((int)s);
+    if (fin.empty()) return;
+    for (int s : fin) {
+        const Slot &S = slots[s];
+        if (S.x_out)
+            HS_CHECK_HIP(hipMemcpyAsync(S.x_out, arena_dev + S.off + S.lay.o_x, S.lay.n * 8,
+                                        hipMemcpyDeviceToHost, cs));
+    }
+    HS_CHECK_HIP(hipStreamSynchronize(cs));
+    const double t = now();
+    for (int s : fin) {
+        Slot &S = slots[s];
+        const CGDev &c = h_out[s].cg;
+        hs_batch_result r{};
+        r.id = S.id;
+        r.n = S.lay.n;
+        r.iters = c.iter;
+        r.status = c.status;
+        r.rnorm0 = c.rnorm0;
+        r.rnorm = c.rnorm;
+        r.true_relres = c.b2 > 0 ? std::sqrt(c.res2 / c.b2) : std::sqrt(c.res2);
+        r.t_submit = S.t_submit;
+        r.t_start = S.t_start;
+        r.t_done = t;
+        results.push_back(r);
+        arena.release(S.off, S.lay.total);
+        S.state = SLOT_FREE;
+        S.x_out = nullptr;
+        free_slots.push_back(s);
+        ++retired;
+    }
+    tables_dirty = true;
+}
+
+}  // namespace batch
+}  // namespace hs
+
+struct hs_batch : hs::batch::Engine {};
+
+using namespace hs;
+
+extern "C" {
+
+int hs_batch_create(const hs_batch_config *cfg, hs_batch_t *out) {
+    return guard([&] {
+        HS_REQUIRE(cfg && out, HS_EINVAL, "NULL argument");
+        *out = nullptr;
+        HS_REQUIRE(cfg->arena_bytes > 0, HS_EINVAL, "arena_bytes must be positive");
+        HS_REQUIRE(cfg->max_systems > 0 && cfg->max_systems <= (1 << 20), HS_EINVAL,
+                   "max_systems must be in [1, 2^20]");
+        HS_REQUIRE(cfg->staging_bytes > 0, HS_EINVAL, "staging_bytes must be positive");
+        HS_REQUIRE(cfg->partsize == 16 || cfg->partsize == 32 || cfg->partsize == 64,
+                   HS_EUNSUPPORTED, "the batched engine takes partsize 16, 32 or 64");
+        HS_REQUIRE(cfg->precision == HS_PREC_FP64 || cfg->precision == HS_PREC_FP32 ||
+                       cfg->precision == HS_PREC_BF16,
+                   HS_EINVAL, "unknown precision");
+        HS_REQUIRE(cfg->passes >= 1 && cfg->passes <= 1024, HS_EINVAL,
+                   "passes must be in [1, 1024]");
+        HS_REQUIRE(cfg->grid >= 0, HS_EINVAL, "grid must be >= 0");
+        std::unique_ptr<hs_batch> e(new hs_batch());
+        e->create(*cfg);
+        *out = e.release();
+    });
+}
+
+int hs_batch_footprint(hs_batch_t e, int64_t n, const int64_t *ptr, int64_t *bytes) {
+    return guard([&] {
+        HS_REQUIRE(e && ptr && bytes, HS_EINVAL, "NULL argument");
+        HS_REQUIRE(n > 0, HS_EINVAL, "system size must be positive");
+        *bytes = batch::make_layout(n, ptr, e->cfg.partsize, e->tsz, e->dsz).total;
+    });
+}
+
+int hs_batch_submit(hs_batch_t e, int64_t n, const int64_t *ptr, const int64_t *ind,
+                    const double *val, const double *b, double *x_out, double rtol, double atol,
+                    int64_t itmax, int64_t *id) {
+    int status = HS_OK;
+    int rc = guard([&] {
+        HS_REQUIRE(e && ptr && ind && val && b && id, HS_EINVAL, "NULL argument");
+        status = e->submit(n, ptr, ind, val, b, x_out, rtol, atol, itmax < 0 ? 2 * n : itmax, id);
+        if (status == HS_EAGAIN) set_error(HS_EAGAIN, "no free slot or arena space; retry later");
+    });
+    return rc != HS_OK ? rc : status;
+}
+
+int hs_batch_step(hs_batch_t e) {
+    return guard([&] {
+        HS_REQUIRE(e, HS_EINVAL, "NULL engine");
+        e->step();
+    });
+}
+
+int hs_batch_collect(hs_batch_t e, hs_batch_result *out, int64_t max, int64_t *count) {
+    return guard([&] {
+        HS_REQUIRE(e && count && (max == 0 || out), HS_EINVAL, "NULL argument");
+        e->collect();
+        int64_t k = 0;
+        while (k < max && !e->results.empty()) {
+            out[k++] = e->results.front();
+            e->results.pop_front();
+        }
+        *count = k;
+    });
+}
+
+int hs_batch_info(hs_batch_t e, int64_t *out) {
+    return guard([&] {
+        HS_REQUIRE(e && out, HS_EINVAL, "NULL argument");
+        int64_t active = 0, uploading = 0;
+        for (auto &s : e->slots) {
+            active += s.state == batch::SLOT_ACTIVE;
+            uploading += s.state == batch::SLOT_UPLOADING;
+        }
+        out[0] = active;
+        out[1] = uploading;
+        out[2] = (int64_t)e->free_slots.size();
+        out[3] = e->arena.capacity() - e->arena.used();
+        out[4] = e->arena.capacity();
+        out[5] = e->arena.largest();
+        out[6] = e->steps;
+        out[7] = e->retired;
+        out[8] = (int64_t)e->results.size();
+        out[9] = e->grid;
+    });
+}
+
+int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out) {
+    return guard([&] {
+        HS_REQUIRE(e && out, HS_EINVAL, "NULL argument");
+        HS_REQUIRE(what >= 0 && what <= 6, HS_EINVAL, "unknown quantity");
+        e->wait_step();
+        HS_CHECK_HIP(hipStreamSynchronize(e->us));
+        const batch::Slot *S = nullptr;
+        for (const auto &s : e->slots)
+            if (s.state != batch::SLOT_FREE && s.id == id) S = &s;
+        HS_REQUIRE(S, HS_EINVAL, "no resident system with this id");
+        const batch::Layout &L = S->lay;
+        const uint8_t *blk = e->arena_dev + S->off;
+        const int64_t off[5] = {L.o_x, L.o_r, L.o_z, L.o_b, L.o_q};
+        if (what <= 4) {
+            HS_CHECK_HIP(hipMemcpy(out, blk + off[what], L.n * 8, hipMemcpyDeviceToHost));
+        } else if (what == 5) {  // D_Dl1 widened to FP64
+            if (e->dsz == 8) {
+                HS_CHECK_HIP(hipMemcpy(out, blk + L.o_D, L.n * 8, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> f(L.n);
+                HS_CHECK_HIP(hipMemcpy(f.data(), blk + L.o_D, L.n * 4, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < L.n; ++i) out[i] = f[i];
+            }
+        } else {  // the two partials of every tile
+            HS_CHECK_HIP(hipMemcpy(out, blk + L.o_part, 2 * L.ntiles * 8, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+double hs_batch_clock(hs_batch_t e) { return e ? e->now() : 0.0; }
+
+int hs_batch_destroy(hs_batch_t e) {
+    return guard([&] { delete e; });
+}
+
+}  // extern "C"

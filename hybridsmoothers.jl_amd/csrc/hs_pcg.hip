@@ -52,20 +52,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_cg_fin0(const double *__restrict_
                                                        unsigned count, CGDev *st) {
     __shared__ double sm[16];
     double tot = finalize_sum(partials, count, 1, sm);
-    if (threadIdx.x == 0) {
-        st->gamma = tot;
-        double rn = sqrt(tot);
-        st->rnorm0 = rn;
-        st->rnorm = rn;
-        st->eps = st->atol + st->rtol * rn;
-        st->iter = 0;
-        st->status = 0;
-        st->done = (tot == 0.0 || rn <= st->eps) ? 1 : 0;
-        if (!st->done && st->itmax <= 0) {
-            st->done = 1;
-            st->status = 1;
-        }
-    }
+    if (threadIdx.x == 0) st->done = cg_scalars_start(*st, tot) ? 1 : 0;
 }
 
 // q = A p; delta = p'q
@@ -92,13 +79,8 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_cg_fin_delta(const double *__rest
     __shared__ double sm[16];
     if (st->done) return;
     double tot = finalize_sum(partials, count, 1, sm);
-    if (threadIdx.x == 0) {
-        st->delta = tot;
-        if (!(tot > 0.0)) {  // breakdown: A (or M) not positive definite
-            st->done = 1;
-            st->status = 2;
-        }
-    }
+    // breakdown (p'q <= 0): A or M is not positive definite
+    if (threadIdx.x == 0 && !cg_scalars_delta(*st, tot)) st->done = 1;
 }
 
 // Fused CG form with an L1GS preconditioner, iteration k (st->iter == k):
@@ -120,21 +102,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv_fused(int64_t n, SellView<doubl
     double *pnew = (k & 1) ? pb : pa;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.0;
-    if (i < n) {
-        const int64_t s = i >> 6;
-        const int l = (int)(i & 63);
-        const int64_t e = A.sptr[s + 1];
-        double acc = 0.0;
-        for (int64_t p = A.sptr[s] + l; p < e; p += 64) {
-            const int32_t c = A.col[p];
-            const double pc = first ? z[c] : z[c] + beta * pold[c];
-            acc = acc + A.val[p] * pc;
-        }
-        const double pi = first ? z[i] : z[i] + beta * pold[i];
-        pnew[i] = pi;
-        q[i] = acc;
-        d = pi * acc;
-    }
+    if (i < n) d = cg_spmv_row(A, z, pold, pnew, q, first, beta, i);
     double bs = block_sum(d, sm);
     if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
@@ -169,24 +137,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_cg_fin_gamma(const double *__rest
     __shared__ double sm[16];
     if (st->done) return;
     double tot = finalize_sum(partials, count, 1, sm);
-    if (threadIdx.x == 0) {
-        st->gnext = tot;
-        double rn = sqrt(tot);
-        st->rnorm = rn;
-        st->iter += 1;
-        bool solved = (rn <= st->eps) || (rn + 1.0 <= 1.0);
-        if (solved) {
-            st->done = 1;
-            st->status = 0;
-        } else {
-            st->beta = tot / st->gamma;
-            st->gamma = tot;
-            if (st->iter >= st->itmax) {
-                st->done = 1;
-                st->status = 1;
-            }
-        }
-    }
+    if (threadIdx.x == 0 && cg_scalars_next(*st, tot)) st->done = 1;
 }
 
 // p = z + beta p

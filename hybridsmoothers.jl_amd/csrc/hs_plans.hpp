@@ -15,6 +15,79 @@ struct CGDev {
     int done, status;
 };
 
+// Scalar steps of the CG recurrence (one thread), shared by the single-system
+// and the batched solvers so that both run the same arithmetic.
+// After gamma0 = r0'z0: sets rnorm0, eps and the trivial-convergence test.
+// Returns true when the solve is finished (st.status says why).
+__device__ __forceinline__ bool cg_scalars_start(CGDev &st, double gamma0) {
+    st.gamma = gamma0;
+    const double rn = sqrt(gamma0);
+    st.rnorm0 = rn;
+    st.rnorm = rn;
+    st.eps = st.atol + st.rtol * rn;
+    st.iter = 0;
+    st.status = 0;
+    bool done = (gamma0 == 0.0 || rn <= st.eps);
+    if (!done && st.itmax <= 0) {
+        done = true;
+        st.status = 1;
+    }
+    return done;
+}
+
+// After gamma' = r'z of an iteration: Krylov.jl's stopping rule
+// sqrt(r'z) <= eps, then beta = gamma'/gamma and the iteration limit.
+__device__ __forceinline__ bool cg_scalars_next(CGDev &st, double gnext) {
+    st.gnext = gnext;
+    const double rn = sqrt(gnext);
+    st.rnorm = rn;
+    st.iter += 1;
+    if ((rn <= st.eps) || (rn + 1.0 <= 1.0)) {
+        st.status = 0;
+        return true;
+    }
+    st.beta = gnext / st.gamma;
+    st.gamma = gnext;
+    if (st.iter >= st.itmax) {
+        st.status = 1;
+        return true;
+    }
+    return false;
+}
+
+// delta = p'q; false (breakdown, status 2) when it is not positive.
+__device__ __forceinline__ bool cg_scalars_delta(CGDev &st, double delta) {
+    st.delta = delta;
+    if (!(delta > 0.0)) {
+        st.status = 2;
+        return false;
+    }
+    return true;
+}
+
+// Row i of the fused SpMV of CG iteration k: p_k = z + beta p_{k-1} (p_0 = z)
+// formed on the fly for row i and its columns, q_i = (A p_k)_i in row order.
+// Stores p_k[i] and q[i]; returns p_k[i] * q_i (the p'q partial of the row).
+__device__ __forceinline__ double cg_spmv_row(const SellView<double> &A,
+                                              const double *__restrict__ z,
+                                              const double *__restrict__ pold,
+                                              double *__restrict__ pnew, double *__restrict__ q,
+                                              bool first, double beta, int64_t i) {
+    const int64_t s = i >> 6;
+    const int l = (int)(i & 63);
+    const int64_t e = A.sptr[s + 1];
+    double acc = 0.0;
+    for (int64_t p = A.sptr[s] + l; p < e; p += 64) {
+        const int32_t c = A.col[p];
+        const double pc = first ? z[c] : z[c] + beta * pold[c];
+        acc = acc + A.val[p] * pc;
+    }
+    const double pi = first ? z[i] : z[i] + beta * pold[i];
+    pnew[i] = pi;
+    q[i] = acc;
+    return pi * acc;
+}
+
 // CG vector updates fused into a symmetric L1GS sweep: before the sweep each
 // row does r -= alpha q and x += alpha p (alpha = gamma/delta), the sweep
 // reads the updated r, and afterwards each block stores its partial of r'z

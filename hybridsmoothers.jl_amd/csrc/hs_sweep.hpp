@@ -1,0 +1,326 @@
+// hs_sweep.hpp -- device building blocks of the partitioned L1GS sweeps,
+// shared by the single-system kernels (hs_l1gs.hip) and the batched engine
+// (hs_batch.hip): packed-block layout, the D_Dl1 rule, segment broadcasts,
+// correctly rounded division from a reciprocal, and the per-partition
+// right-looking recurrences.  The layout and recurrence are described at the
+// top of hs_l1gs.hip.
+#pragma once
+
+#include "hs_plans.hpp"
+
+#include <cmath>
+#include <type_traits>
+
+namespace hs {
+
+enum { MODE_FWD = 0, MODE_BWD = 1, MODE_SYM = 2 };
+
+// ---------------------------------------------------------------------------
+// layout helpers
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int64_t colL_off(int64_t j, int64_t Pa) {
+    return j * (Pa - 1) - (j * (j - 1)) / 2;
+}
+__host__ __device__ __forceinline__ int64_t colU_off(int64_t j, int64_t Pa) {
+    return (Pa * (Pa - 1) - j * (j + 1)) / 2;
+}
+// elements per partition block of the device buffer
+__host__ __device__ __forceinline__ int64_t blk_stride(int64_t P, int mode) {
+    return mode == MODE_SYM ? P * (P - 1) : (P * (P - 1)) / 2;
+}
+// start of the lower and upper triangles of partition k (Pa rows)
+__host__ __device__ __forceinline__ void block_bases(int64_t k, int64_t P, int64_t Pa, int mode,
+                                                     int64_t &lb, int64_t &ub) {
+    lb = k * blk_stride(P, mode);
+    ub = (mode == MODE_SYM) ? lb + (Pa * (Pa - 1)) / 2 : lb;
+}
+
+__host__ __device__ __forceinline__ double dl1_rule(double a, double d, double eta) {
+    // _compute_D_Dl1_kernel! (:1016-1022): a_ii + (a_ii >= eta*d ? 0 : d/2)
+    double star = (a >= eta * d) ? 0.0 : d / 2.0;
+    return a + star;
+}
+
+
+// ---------------------------------------------------------------------------
+// sweep kernels
+// ---------------------------------------------------------------------------
+// Broadcast of lane j of every PM-lane segment to the whole segment.
+// PM = 64: v_readlane into SGPRs; PM = 32: readlanes of lanes j and j+32 and a
+// per-half select; smaller segments: ds_bpermute (__shfl with width PM).
+template <int PM>
+__device__ __forceinline__ double seg_bcast(double v, int j) {
+    if constexpr (PM == 64) {
+        int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+        int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+        return __hiloint2double(hi, lo);
+    } else if constexpr (PM == 32) {
+        int lo0 = __builtin_amdgcn_readlane(__double2loint(v), j);
+        int hi0 = __builtin_amdgcn_readlane(__double2hiint(v), j);
+        int lo1 = __builtin_amdgcn_readlane(__double2loint(v), j + 32);
+        int hi1 = __builtin_amdgcn_readlane(__double2hiint(v), j + 32);
+        bool up = (threadIdx.x & 32) != 0;
+        return __hiloint2double(up ? hi1 : hi0, up ? lo1 : lo0);
+    } else {
+        return __shfl(v, j, PM);
+    }
+}
+template <int PM>
+__device__ __forceinline__ float seg_bcast(float v, int j) {
+    if constexpr (PM == 64) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+    } else if constexpr (PM == 32) {
+        int a = __builtin_amdgcn_readlane(__float_as_int(v), j);
+        int b = __builtin_amdgcn_readlane(__float_as_int(v), j + 32);
+        return __int_as_float((threadIdx.x & 32) ? b : a);
+    } else {
+        return __shfl(v, j, PM);
+    }
+}
+
+// v_writelane_b32 (no clang builtin exists; this binds the LLVM intrinsic).
+extern "C" __device__ int hs_writelane_i32(int value, int lane, int old)
+    __asm("llvm.amdgcn.writelane.i32");
+
+// Broadcast lane j of every segment and record the broadcast value in lane j
+// of `keep` (v_writelane from the SGPR copy: no per-step lane masks).
+template <int PM>
+__device__ __forceinline__ double bcast_keep(double v, int j, double &keep) {
+    if constexpr (PM == 64) {
+        int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+        int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+        int klo = hs_writelane_i32(lo, j, __double2loint(keep));
+        int khi = hs_writelane_i32(hi, j, __double2hiint(keep));
+        keep = __hiloint2double(khi, klo);
+        return __hiloint2double(hi, lo);
+    } else if constexpr (PM == 32) {
+        int lo0 = __builtin_amdgcn_readlane(__double2loint(v), j);
+        int hi0 = __builtin_amdgcn_readlane(__double2hiint(v), j);
+        int lo1 = __builtin_amdgcn_readlane(__double2loint(v), j + 32);
+        int hi1 = __builtin_amdgcn_readlane(__double2hiint(v), j + 32);
+        int klo = hs_writelane_i32(lo0, j, __double2loint(keep));
+        int khi = hs_writelane_i32(hi0, j, __double2hiint(keep));
+        klo = hs_writelane_i32(lo1, j + 32, klo);
+        khi = hs_writelane_i32(hi1, j + 32, khi);
+        keep = __hiloint2double(khi, klo);
+        bool up = (threadIdx.x & 32) != 0;
+        return __hiloint2double(up ? hi1 : hi0, up ? lo1 : lo0);
+    } else {
+        const int lane = threadIdx.x % PM;
+        keep = (lane == j) ? v : keep;
+        return __shfl(v, j, PM);
+    }
+}
+template <int PM>
+__device__ __forceinline__ float bcast_keep(float v, int j, float &keep) {
+    if constexpr (PM == 64) {
+        int b = __builtin_amdgcn_readlane(__float_as_int(v), j);
+        keep = __int_as_float(hs_writelane_i32(b, j, __float_as_int(keep)));
+        return __int_as_float(b);
+    } else if constexpr (PM == 32) {
+        int a = __builtin_amdgcn_readlane(__float_as_int(v), j);
+        int b = __builtin_amdgcn_readlane(__float_as_int(v), j + 32);
+        int k = hs_writelane_i32(a, j, __float_as_int(keep));
+        keep = __int_as_float(hs_writelane_i32(b, j + 32, k));
+        return __int_as_float((threadIdx.x & 32) ? b : a);
+    } else {
+        const int lane = threadIdx.x % PM;
+        keep = (lane == j) ? v : keep;
+        return __shfl(v, j, PM);
+    }
+}
+
+// Correctly rounded z/d from the correctly rounded reciprocal r = 1/d:
+// q0 = z*r, e = z - q0*d (exact with an FMA), q = q0 + e*r rounds to z/d
+// (Markstein's theorem) whenever d, r and z/d are normal numbers -- the
+// sweep checks |d| once per partition and otherwise uses true division.
+__device__ __forceinline__ double quot_fast(double z, double d, double r) {
+    double q0 = z * r;
+    double e = __builtin_fma(-q0, d, z);
+    return __builtin_fma(e, r, q0);
+}
+__device__ __forceinline__ float quot_fast(float z, float d, float r) {
+    float q0 = z * r;
+    float e = __builtin_fmaf(-q0, d, z);
+    return __builtin_fmaf(e, r, q0);
+}
+template <bool FAST, class TC>
+__device__ __forceinline__ TC quot(TC z, TC d, TC r) {
+    if constexpr (FAST) return quot_fast(z, d, r);
+    else return z / d;
+}
+__device__ __forceinline__ bool safe_divisor(double d) {
+    double a = fabs(d);
+    return a >= 0x1p-400 && a <= 0x1p400;
+}
+__device__ __forceinline__ bool safe_divisor(float d) {
+    float a = fabsf(d);
+    return a >= 0x1p-60f && a <= 0x1p60f;
+}
+
+// Any partition shape (last, short partition; P below the segment width):
+// steps guarded by the actual size Pa.
+template <int PM, int MODE, bool FAST, class TS, class TC>
+__device__ __forceinline__ TC sweep_generic(TC xr, TC dr, TC rr, int lane, int Pa, bool live,
+                                            const TS *__restrict__ Lk,
+                                            const TS *__restrict__ Uk) {
+    TC yr = xr;
+    if constexpr (MODE != MODE_BWD) {
+        TC acc = (TC)0, yfin = (TC)0;
+#pragma unroll
+        for (int j = 0; j < PM; ++j) {
+            if (j < Pa) {
+                TC q = quot<FAST>(xr - acc, dr, rr);
+                yfin = (lane == j) ? q : yfin;
+                if (j < Pa - 1) {
+                    TC yj = seg_bcast<PM>(q, j);
+                    bool has = (lane > j) && live;
+                    TC l = has ? load_as<TC>(Lk[colL_off(j, Pa) + (lane - j - 1)]) : (TC)0;
+                    if (lane > j) acc = acc + l * yj;
+                }
+            }
+        }
+        yr = yfin;
+    }
+    if constexpr (MODE == MODE_SYM) xr = dr * yr;  // y .= D_Dl1 .* y  (:897)
+    if constexpr (MODE != MODE_FWD) {
+        TC acc = (TC)0, yfin = (TC)0;
+#pragma unroll
+        for (int jj = 0; jj < PM; ++jj) {
+            const int j = PM - 1 - jj;
+            if (j < Pa) {
+                TC q = quot<FAST>(xr - acc, dr, rr);
+                yfin = (lane == j) ? q : yfin;
+                if (j > 0) {
+                    TC yj = seg_bcast<PM>(q, j);
+                    TC u = (lane < j) ? load_as<TC>(Uk[colU_off(j, Pa) + lane]) : (TC)0;
+                    if (lane < j) acc = acc + u * yj;
+                }
+            }
+        }
+        yr = yfin;
+    }
+    return yr;
+}
+
+// Full partitions (Pa == PM): straight-line, branch-free recurrence.  The
+// column loads of both passes form one stream that runs RING columns ahead
+// of the step consuming them, so every wave keeps RING*64 values in flight
+// while it computes and the compiler can wait on exact load counts.
+template <int PM, int MODE, int RING, class TS, class TC>
+__device__ __forceinline__ TC sweep_full(TC xr, TC dr, TC rr, int lane,
+                                         const TS *__restrict__ Lk, const TS *__restrict__ Uk) {
+    constexpr int NL = (MODE != MODE_BWD) ? PM - 1 : 0;  // forward columns 0..PM-2
+    constexpr int NU = (MODE != MODE_FWD) ? PM - 1 : 0;  // backward columns PM-1..1
+    constexpr int NC = NL + NU;
+    constexpr int R = RING < NC ? RING : NC;
+    // Unconditional loads: lanes whose row is already final (i <= j forward,
+    // i >= j backward) read a neighbouring in-block value into an accumulator
+    // that is never used again.  Indices stay inside the triangle: forward
+    // column j reads colL(j) + lane - j - 1 >= 0 for j >= 1 (clamped for j = 0);
+    // backward column j reads colU(j) + lane, clamped to the column for the
+    // last stored columns (small j), whose tail would pass the triangle's end.
+    auto load_col = [&](int c) -> TS {
+        if (c < NL) {
+            const int j = c;
+            const int off = (j == 0) ? (lane > 0 ? lane - 1 : 0) : lane - j - 1;
+            return Lk[colL_off(j, PM) + off];
+        }
+        const int j = PM - 1 - (c - NL);
+        constexpr int64_t UEND = (PM * (PM - 1)) / 2;
+        const bool tail = colU_off(j, PM) + PM > UEND;
+        const int off = tail ? (lane < j - 1 ? lane : j - 1) : lane;
+        return Uk[colU_off(j, PM) + off];
+    };
+    TS ring[R > 0 ? R : 1];
+#pragma unroll
+    for (int c = 0; c < R; ++c) ring[c] = load_col(c);
+
+    TC yr = xr;
+    if constexpr (MODE != MODE_BWD) {
+        TC acc = (TC)0, yfin = (TC)0;
+#pragma unroll
+        for (int j = 0; j < PM - 1; ++j) {
+            const TS v = ring[j % R];
+            if (j + R < NC) ring[j % R] = load_col(j + R);
+            TC q = quot_fast(xr - acc, dr, rr);
+            TC yj = bcast_keep<PM>(q, j, yfin);
+            acc = acc + load_as<TC>(v) * yj;  // dead for lanes <= j
+        }
+        TC q = quot_fast(xr - acc, dr, rr);
+        yr = (lane == PM - 1) ? q : yfin;
+    }
+    if constexpr (MODE == MODE_SYM) xr = dr * yr;  // y .= D_Dl1 .* y  (:897)
+    if constexpr (MODE != MODE_FWD) {
+        TC acc = (TC)0, yfin = (TC)0;
+#pragma unroll
+        for (int jj = 0; jj < PM - 1; ++jj) {
+            const int j = PM - 1 - jj;
+            const int c = NL + jj;
+            const TS v = ring[c % R];
+            if (c + R < NC) ring[c % R] = load_col(c + R);
+            TC q = quot_fast(xr - acc, dr, rr);
+            TC yj = bcast_keep<PM>(q, j, yfin);
+            acc = acc + load_as<TC>(v) * yj;  // dead for lanes >= j
+        }
+        TC q = quot_fast(xr - acc, dr, rr);
+        yr = (lane == 0) ? q : yfin;
+    }
+    return yr;
+}
+
+
+// Row prologue of the fused CG form: r -= alpha q; x += alpha p (the k_cg_axpy
+// arithmetic); returns the updated r, the sweep's input.
+__device__ __forceinline__ double cg_prologue(const CGFuse &f, int64_t i) {
+    const double alpha = f.st->gamma / f.st->delta;
+    const double *p = (f.st->iter & 1) ? f.pb : f.pa;
+    const double ri = f.r[i] - alpha * f.q[i];
+    f.r[i] = ri;
+    f.x[i] = f.x[i] + alpha * p[i];
+    return ri;
+}
+
+
+// One partition per PM-lane segment; returns r_i * z_i of the lane's row
+// (fused CG form, or x_i * y_i) or 0.  Every lane of the wave must call it
+// together: the full-partition recurrence moves values between the segments
+// of a wave (v_readlane / v_writelane, which ignore the execution mask), so
+// the choice of recurrence is made once per wave, and a segment past the last
+// partition (k >= nparts) runs the last partition's recurrence on dead lanes
+// -- its loads stay inside the buffers and it stores nothing.
+template <int PM, int MODE, class TS, class TC, bool CG>
+__device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
+                                                 const TS *__restrict__ T,
+                                                 const TC *__restrict__ D, int64_t n, int P,
+                                                 int64_t nparts, int64_t k, int lane,
+                                                 const CGFuse &f) {
+    const bool valid = k < nparts;
+    const int64_t kk = valid ? k : nparts - 1;
+    const int64_t s = kk * (int64_t)P;
+    const int64_t rem = n - s;
+    const int Pa = (int)(rem < P ? rem : P);
+    int64_t lb, ub;
+    block_bases(kk, P, Pa, MODE, lb, ub);
+    const bool live = valid && lane < Pa;
+    const TS *Lk = T + lb;
+    const TS *Uk = T + ub;
+
+    double xin = 0.0;
+    if (live) xin = CG ? cg_prologue(f, s + lane) : x[s + lane];
+    const TC xr = (TC)xin;
+    const TC dr = live ? D[s + lane] : (TC)1;
+    const TC rr = (TC)1 / dr;
+    TC yr;
+    if (__any(!safe_divisor(dr))) {
+        yr = sweep_generic<PM, MODE, false, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
+    } else if (__all(Pa == PM)) {
+        yr = sweep_full<PM, MODE, 16, TS, TC>(xr, dr, rr, lane, Lk, Uk);
+    } else {
+        yr = sweep_generic<PM, MODE, true, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
+    }
+    if (live) y[s + lane] = (double)yr;
+    return live ? xin * (double)yr : 0.0;
+}
+
+}  // namespace hs

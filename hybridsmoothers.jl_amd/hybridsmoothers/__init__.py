@@ -16,6 +16,7 @@ from .l1gs import (  # noqa: F401
 )
 from .l1gs import ldiv_ as _l1gs_ldiv_
 from .pcg import DeviceMatrix, KrylovJL_CG, LinearProblem, Solution, pcg_solve, solve  # noqa: F401
+from .batch import BatchSolution, BatchSolver  # noqa: F401
 
 __all__ = [
     "L1GSPrecBuilder", "ForwardSweep", "BackwardSweep", "SymmetricSweep",

@@ -21,6 +21,7 @@ HS_EDEVICE = -3
 HS_ENOMEM = -4
 HS_ENODEVICE = -5
 HS_EUNSUPPORTED = -6
+HS_EAGAIN = -7
 
 FMT_CSC, FMT_CSR = 0, 1
 SWEEP_FORWARD, SWEEP_BACKWARD, SWEEP_SYMMETRIC = 0, 1, 2
@@ -58,6 +59,22 @@ class PCGStats(C.Structure):
                 ("rnorm", C.c_double), ("true_relres", C.c_double), ("seconds", C.c_double)]
 
 
+class BatchConfig(C.Structure):
+    """hs_batch_config"""
+    _fields_ = [("arena_bytes", C.c_int64), ("max_systems", C.c_int64),
+                ("staging_bytes", C.c_int64), ("partsize", C.c_int64),
+                ("precision", C.c_int), ("passes", C.c_int), ("grid", C.c_int),
+                ("eta", C.c_double)]
+
+
+class BatchResult(C.Structure):
+    """hs_batch_result"""
+    _fields_ = [("id", C.c_int64), ("n", C.c_int64), ("iters", C.c_int64),
+                ("status", C.c_int64), ("rnorm0", C.c_double), ("rnorm", C.c_double),
+                ("true_relres", C.c_double), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
 def _sig(name, args, res=C.c_int):
     f = getattr(lib, name)
     f.argtypes = args
@@ -84,6 +101,9 @@ EXPORTED = [
     "hs_csr_create", "hs_csr_spmv", "hs_csr_info", "hs_csr_get", "hs_csr_destroy",
     "hs_cheb_build", "hs_cheb_smooth", "hs_cheb_apply", "hs_cheb_info", "hs_cheb_get",
     "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve",
+    "hs_problem_lognormal",
+    "hs_batch_create", "hs_batch_footprint", "hs_batch_submit", "hs_batch_step",
+    "hs_batch_collect", "hs_batch_info", "hs_batch_clock", "hs_batch_peek", "hs_batch_destroy",
 ]
 
 
@@ -107,6 +127,18 @@ def bind_optional():
     if hasattr(lib, "hs_pcg_solve"):
         _sig("hs_pcg_solve", [vp, C.c_int, vp, vp, vp, C.c_double, C.c_double, i64, C.c_int, vp,
                               C.POINTER(PCGStats)])
+    if hasattr(lib, "hs_problem_lognormal"):
+        _sig("hs_problem_lognormal", [i64, i64, i64, C.c_double, C.c_uint64, vp, vp, vp, vp])
+    if hasattr(lib, "hs_batch_create"):
+        _sig("hs_batch_create", [C.POINTER(BatchConfig), C.POINTER(vp)])
+        _sig("hs_batch_footprint", [vp, i64, vp, vp])
+        _sig("hs_batch_submit", [vp, i64, vp, vp, vp, vp, vp, C.c_double, C.c_double, i64, vp])
+        _sig("hs_batch_step", [vp])
+        _sig("hs_batch_collect", [vp, vp, i64, vp])
+        _sig("hs_batch_info", [vp, vp])
+        _sig("hs_batch_clock", [vp], C.c_double)
+        _sig("hs_batch_peek", [vp, i64, C.c_int, vp])
+        _sig("hs_batch_destroy", [vp])
 
 
 bind_optional()

@@ -1,0 +1,200 @@
+"""Batched persistent PCG over many SPD systems (host side of hs_batch_*).
+
+No counterpart in HybridSmoothers.jl, whose users solve one system per
+`solve(LinearProblem(A, b), KrylovJL_CG(); Pl = P)` call (README.md:27-29).
+Each system submitted here is solved exactly as
+
+    P = L1GSPrecBuilder(ROCBackend(), ...)(A, partsize, isSymA=True, sweep=SymmetricSweep(),
+                                           precision=precision)
+    pcg_solve(DeviceMatrix(A), b, Pl=P, reltol=rtol, abstol=atol)
+
+would solve it (FP64 CG, Krylov.jl's stopping rule, x0 = 0), but many systems
+share every kernel launch.  The engine keeps matrices, preconditioners and CG
+state resident in one device arena; `submit` returns None when the arena or
+the slot table is full (admission by device-memory footprint).
+
+    eng = BatchSolver(partsize=32, precision="fp32")
+    ids = [eng.submit(A, b) for A, b in systems]     # None: retry after collect()
+    while eng.pending():
+        eng.step()                                    # one graph launch, async
+        for sol in eng.collect():                     # waits, retires finished
+            ...
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib as L
+
+_PREC = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32, "bf16": L.PREC_BF16}
+_STATUS = {0: "solved", 1: "maxiters", 2: "breakdown"}
+
+
+@dataclass
+class BatchSolution:
+    id: int
+    x: np.ndarray
+    iters: int
+    status: str
+    rnorm0: float
+    rnorm: float
+    true_relres: float
+    t_submit: float
+    t_start: float
+    t_done: float
+    tag: Any = None
+
+
+def csr_arrays(A):
+    """1-based Int64 (rowptr, colval) and Float64 values of a symmetric matrix.
+
+    A scipy CSR or CSC matrix (a symmetric CSC's columns are its rows) or a
+    tuple of 1-based arrays (ptr, ind, val) as produced by problems.lognormal_grid.
+    """
+    if isinstance(A, tuple):
+        ptr, ind, val = A
+        return (np.ascontiguousarray(ptr, dtype=np.int64), np.ascontiguousarray(ind, dtype=np.int64),
+                np.ascontiguousarray(val, dtype=np.float64))
+    if not (sp.isspmatrix_csr(A) or sp.isspmatrix_csc(A)):
+        A = sp.csr_matrix(A)
+    if not A.has_sorted_indices:
+        A = A.sorted_indices()
+    if A.shape[0] != A.shape[1]:
+        raise L.HSError(L.HS_EINVAL, "matrix must be square")
+    return (A.indptr.astype(np.int64) + 1, A.indices.astype(np.int64) + 1,
+            np.ascontiguousarray(A.data, dtype=np.float64))
+
+
+def default_arena_bytes(fraction: float = 0.5, cap: int = 96 << 30) -> int:
+    try:
+        import torch
+        free, _ = torch.cuda.mem_get_info()
+        return int(min(free * fraction, cap))
+    except Exception:  # pragma: no cover
+        return 8 << 30
+
+
+class BatchSolver:
+    def __init__(self, partsize: int = 32, precision: str = "fp32", arena_bytes: int | None = None,
+                 max_systems: int = 1024, staging_bytes: int = 256 << 20, passes: int = 8,
+                 grid: int = 0, eta: float = 1.5, device: int = 0):
+        L.init(device)
+        if precision not in _PREC:
+            raise L.HSError(L.HS_EINVAL, f"unknown precision {precision!r}")
+        cfg = L.BatchConfig(arena_bytes=int(arena_bytes or default_arena_bytes()),
+                            max_systems=int(max_systems), staging_bytes=int(staging_bytes),
+                            partsize=int(partsize), precision=_PREC[precision],
+                            passes=int(passes), grid=int(grid), eta=float(eta))
+        self.config = cfg
+        self.partsize, self.precision, self.passes = int(partsize), precision, int(passes)
+        self._h = C.c_void_p()
+        L.check(L.lib.hs_batch_create(C.byref(cfg), C.byref(self._h)))
+        self._live: dict[int, tuple[np.ndarray, Any]] = {}  # id -> (x buffer, tag)
+        self._res = (L.BatchResult * 256)()
+
+    # -- admission ---------------------------------------------------------
+    def footprint(self, A) -> int:
+        ptr, _, _ = csr_arrays(A)
+        out = C.c_int64(0)
+        L.check(L.lib.hs_batch_footprint(self._h, ptr.size - 1, ptr.ctypes.data, C.byref(out)))
+        return out.value
+
+    def submit(self, A, b, rtol: float = 1e-8, atol: float = 0.0, itmax: int = -1,
+               tag: Any = None) -> int | None:
+        """Uploads one system; its id, or None when the engine is full right now."""
+        ptr, ind, val = csr_arrays(A)
+        n = ptr.size - 1
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.shape != (n,):
+            raise L.HSError(L.HS_EINVAL, f"b must have length {n}")
+        x = np.empty(n, np.float64)
+        sid = C.c_int64(-1)
+        st = L.lib.hs_batch_submit(self._h, n, ptr.ctypes.data, ind.ctypes.data, val.ctypes.data,
+                                   b.ctypes.data, x.ctypes.data, float(rtol), float(atol),
+                                   int(itmax), C.byref(sid))
+        if st == L.HS_EAGAIN:
+            return None
+        L.check(st)
+        self._live[sid.value] = (x, tag)
+        return sid.value
+
+    # -- progress ------------------------------------------------------------
+    def step(self) -> None:
+        L.check(L.lib.hs_batch_step(self._h))
+
+    def collect(self) -> list[BatchSolution]:
+        out = []
+        cnt = C.c_int64(0)
+        while True:
+            L.check(L.lib.hs_batch_collect(self._h, self._res, len(self._res), C.byref(cnt)))
+            for k in range(cnt.value):
+                r = self._res[k]
+                x, tag = self._live.pop(r.id)
+                out.append(BatchSolution(r.id, x, int(r.iters), _STATUS.get(int(r.status), "?"),
+                                         r.rnorm0, r.rnorm, r.true_relres, r.t_submit, r.t_start,
+                                         r.t_done, tag))
+            if cnt.value < len(self._res):
+                return out
+
+    def pending(self) -> int:
+        """Systems submitted and not yet collected."""
+        return len(self._live)
+
+    def info(self) -> dict:
+        v = np.zeros(10, np.int64)
+        L.check(L.lib.hs_batch_info(self._h, v.ctypes.data))
+        keys = ("active", "uploading", "free_slots", "free_bytes", "arena_bytes",
+                "largest_free_block", "steps", "retired", "results_waiting", "grid")
+        return dict(zip(keys, (int(t) for t in v)))
+
+    _PEEK = {"x": 0, "r": 1, "z": 2, "b": 3, "q": 4, "D": 5, "partials": 6}
+
+    def peek(self, sid: int, what: str) -> np.ndarray:
+        """A device vector of a resident system after the last step (diagnostics)."""
+        x, _ = self._live[sid]
+        n = x.size
+        size = 2 * ((n + 255) // 256) if what == "partials" else n
+        out = np.empty(size, np.float64)
+        L.check(L.lib.hs_batch_peek(self._h, sid, self._PEEK[what], out.ctypes.data))
+        return out
+
+    def clock(self) -> float:
+        return L.lib.hs_batch_clock(self._h)
+
+    # -- convenience -----------------------------------------------------------
+    def solve_all(self, systems, rtol: float = 1e-8, atol: float = 0.0, itmax: int = -1):
+        """Solves [(A, b), ...] with continuous admission; solutions in input order."""
+        todo = list(enumerate(systems))
+        done: dict[int, BatchSolution] = {}
+        while todo or self.pending():
+            while todo:
+                k, (A, b) = todo[0]
+                if self.submit(A, b, rtol=rtol, atol=atol, itmax=itmax, tag=k) is None:
+                    if not self.pending():
+                        raise L.HSError(L.HS_ENOMEM, "system does not fit an empty engine")
+                    break
+                todo.pop(0)
+            self.step()
+            for s in self.collect():
+                done[s.tag] = s
+        return [done[k] for k in range(len(systems))]
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            L.lib.hs_batch_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

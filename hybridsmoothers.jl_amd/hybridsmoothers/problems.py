@@ -188,3 +188,104 @@ def serving_stream(count: int, seed: int = 5, nmin: float = 1e3, nmax: float = 2
 def make_system(spec):
     kind, g, s = spec
     return lognormal2d(g, seed=s) if kind == "2d" else lognormal3d(g, seed=s)
+
+
+# ---------------------------------------------------------------------------
+# counter-based lognormal grids (the serving stream's systems)
+# ---------------------------------------------------------------------------
+_M64 = 0xFFFFFFFFFFFFFFFF
+
+
+def _mix64(seed: int, ctr: np.ndarray) -> np.ndarray:
+    """splitmix64 finalizer of seed*phi + ctr*c2 + phi (wrapping uint64)."""
+    with np.errstate(over="ignore"):
+        z = (ctr.astype(np.uint64) * np.uint64(0xD1B54A32D192ED03)
+             + np.uint64(((seed * 0x9E3779B97F4A7C15) + 0x9E3779B97F4A7C15) & _M64))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def lognormal_faces(count: int, seed: int, sigma: float) -> np.ndarray:
+    """k_f = exp(sigma * g_f), g_f standard normal by Box-Muller on counter pairs."""
+    f = np.arange(count, dtype=np.uint64)
+    m = f >> np.uint64(1)
+    h1 = _mix64(seed, np.uint64(2) * m)
+    h2 = _mix64(seed, np.uint64(2) * m + np.uint64(1))
+    two53 = 1.0 / 9007199254740992.0
+    u1 = ((h1 >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * two53
+    u2 = (h2 >> np.uint64(11)).astype(np.float64) * two53
+    r = np.sqrt(-2.0 * np.log(u1))
+    th = 6.283185307179586 * u2
+    g = np.where((f & np.uint64(1)) == 1, r * np.sin(th), r * np.cos(th))
+    return np.exp(sigma * g)
+
+
+def lognormal_grid_numpy(nx: int, ny: int, nz: int = 1, sigma: float = 1.0,
+                         seed: int = 0) -> sp.csc_matrix:
+    """numpy statement of hs_problem_lognormal (symmetric, so CSC = CSR).
+
+    5-point (nz = 1) or 7-point Poisson with face coefficients from
+    lognormal_faces (x faces, then y, then z, each family x fastest); the
+    diagonal is k_xm + k_xp + k_ym + k_yp + k_zm + k_zp with Dirichlet faces,
+    and axes of length 1 have no faces.
+    """
+    hx, hy, hz = nx > 1, ny > 1, nz > 1
+    Fx = (nx + 1) * ny * nz if hx else 0
+    Fy = nx * (ny + 1) * nz if hy else 0
+    Fz = nx * ny * (nz + 1) if hz else 0
+    k = lognormal_faces(Fx + Fy + Fz, seed, sigma)
+    N = nx * ny * nz
+    zero = np.zeros((nz, ny, nx))
+    kx = k[:Fx].reshape(nz, ny, nx + 1) if hx else None
+    ky = k[Fx:Fx + Fy].reshape(nz, ny + 1, nx) if hy else None
+    kz = k[Fx + Fy:].reshape(nz + 1, ny, nx) if hz else None
+    xm = kx[:, :, :-1] if hx else zero
+    xp = kx[:, :, 1:] if hx else zero
+    ym = ky[:, :-1, :] if hy else zero
+    yp = ky[:, 1:, :] if hy else zero
+    zm = kz[:-1, :, :] if hz else zero
+    zp = kz[1:, :, :] if hz else zero
+    diag = (xm + xp + ym + yp + zm + zp).ravel()
+    c = np.arange(N, dtype=np.int64)
+    x = c % nx
+    y = (c // nx) % ny
+    z = c // (nx * ny)
+    offs = [-nx * ny, -nx, -1, 0, 1, nx, nx * ny]
+    coefs = [-zm.ravel(), -ym.ravel(), -xm.ravel(), diag, -xp.ravel(), -yp.ravel(),
+             -zp.ravel()]
+    valid = [z > 0, y > 0, x > 0, np.ones(N, bool), x < nx - 1, y < ny - 1, z < nz - 1]
+    return _assemble_csc(N, offs, coefs, valid)
+
+
+def lognormal_grid(nx: int, ny: int, nz: int = 1, sigma: float = 1.0, seed: int = 0):
+    """The same matrix from libhsmooth's threaded host generator, as 1-based
+    Int64 CSR arrays (rowptr, colval, nzval) -- the arrays a client submits."""
+    import ctypes as C
+    from . import _lib as L
+    nnz = C.c_int64(0)
+    L.check(L.lib.hs_problem_lognormal(nx, ny, nz, sigma, seed, None, None, None, C.byref(nnz)))
+    n = nx * ny * nz
+    rp = np.empty(n + 1, np.int64)
+    ci = np.empty(nnz.value, np.int64)
+    cv = np.empty(nnz.value, np.float64)
+    L.check(L.lib.hs_problem_lognormal(nx, ny, nz, sigma, seed, rp.ctypes.data, ci.ctypes.data,
+                                       cv.ctypes.data, C.byref(nnz)))
+    return rp, ci, cv
+
+
+def csr_to_scipy(rp, ci, cv) -> sp.csr_matrix:
+    n = rp.size - 1
+    return sp.csr_matrix((cv, ci - 1, rp - 1), shape=(n, n))
+
+
+def stream_grid(spec):
+    """(nx, ny, nz) of a serving_stream spec."""
+    kind, g, _ = spec
+    return (g, g, 1) if kind == "2d" else (g, g, g)
+
+
+def stream_system(spec, sigma: float = 1.0):
+    """CSR arrays of a serving_stream spec from the native generator."""
+    nx, ny, nz = stream_grid(spec)
+    return lognormal_grid(nx, ny, nz, sigma=sigma, seed=spec[2])

@@ -44,6 +44,7 @@ extern "C" {
 #define HS_ENOMEM (-4)         /* device allocation failed */
 #define HS_ENODEVICE (-5)      /* backend not functional (no usable GPU) */
 #define HS_EUNSUPPORTED (-6)   /* combination not provided by this library */
+#define HS_EAGAIN (-7)         /* batched engine full right now: retry after collecting */
 
 /* enums */
 #define HS_FMT_CSC 0
@@ -154,6 +155,84 @@ typedef struct {
 int hs_pcg_solve(hs_csr_t A, int prec_kind, void *prec, const double *b, double *x,
                  double rtol, double atol, int64_t itmax, int loc, void *stream,
                  hs_pcg_stats *stats);
+
+/* ---- workload generator -------------------------------------------------- */
+/*
+ * Host generator of the serving workload (no counterpart in the reference;
+ * its tests build 1D/2D stencils in test/test_matrices.jl:5-19): 5-point
+ * (nz = 1) or 7-point variable-coefficient Poisson on nx x ny x nz, face
+ * coefficients exp(sigma * N(0,1)) from a counter-based generator keyed by
+ * `seed`, Dirichlet faces in the diagonal.  Writes 1-based CSR rowptr (n+1),
+ * colval and nzval (nnz); the matrix is symmetric, so these are also its CSC
+ * arrays.  With colval == NULL only *nnz_out is set.
+ */
+int hs_problem_lognormal(int64_t nx, int64_t ny, int64_t nz, double sigma, uint64_t seed,
+                         int64_t *rowptr, int64_t *colval, double *nzval, int64_t *nnz_out);
+
+/* ---- batched persistent PCG (serving engine) ---------------------------- */
+/*
+ * No counterpart in the reference, whose users solve one system per
+ * `solve(LinearProblem(A, b), KrylovJL_CG(); Pl = P)` call (README.md:27-29).
+ * Many SPD systems of uneven size stay resident on the device and advance
+ * together: every graph launch runs `passes` CG iterations of every active
+ * system, each preconditioned by its own symmetric L1GS (isSymA, eta, the
+ * engine's partsize and precision).  Each system's arithmetic is that of
+ * hs_pcg_solve with that preconditioner (x0 = 0, FP64 CG, Krylov.jl's rule),
+ * so its result does not depend on the other systems in the batch.
+ */
+typedef struct hs_batch *hs_batch_t;
+
+typedef struct {
+    int64_t arena_bytes;    /* device memory for resident systems (matrix, smoother, vectors) */
+    int64_t max_systems;    /* resident systems at once */
+    int64_t staging_bytes;  /* pinned host staging for uploads; the largest system must fit */
+    int64_t partsize;       /* 16, 32 or 64 */
+    int precision;          /* smoother storage: HS_PREC_FP64, HS_PREC_FP32 or HS_PREC_BF16 */
+    int passes;             /* CG iterations per step (one graph launch) */
+    int grid;               /* workgroups of the tile kernels; 0 = 8 per CU */
+    double eta;             /* the l1 rule's eta (reference default 1.5) */
+} hs_batch_config;
+
+typedef struct {
+    int64_t id;             /* as returned by hs_batch_submit */
+    int64_t n;
+    int64_t iters;          /* CG iterations */
+    int64_t status;         /* 0 solved, 1 iteration limit, 2 breakdown */
+    double rnorm0, rnorm;   /* sqrt(r0'z0), final sqrt(r'z) */
+    double true_relres;     /* ||b - A x|| / ||b||, FP64 on the device */
+    double t_submit, t_start, t_done;  /* hs_batch_clock() at submit, first step, retirement */
+} hs_batch_result;
+
+int hs_batch_create(const hs_batch_config *cfg, hs_batch_t *out);
+/* Arena bytes a system with this 1-based pointer array (n+1 entries) occupies. */
+int hs_batch_footprint(hs_batch_t e, int64_t n, const int64_t *ptr, int64_t *bytes);
+/*
+ * Admits one symmetric positive definite system: 1-based CSR arrays (or the
+ * CSC arrays of the symmetric matrix, whose columns are its rows), host b.
+ * The upload and the preconditioner build run on the engine's copy stream,
+ * overlapping the steps.  x_out (host, n doubles) receives the solution when
+ * the system is collected and must stay valid until then.  Returns HS_EAGAIN,
+ * changing nothing, when no slot or no arena block is free.
+ */
+int hs_batch_submit(hs_batch_t e, int64_t n, const int64_t *ptr, const int64_t *ind,
+                    const double *val, const double *b, double *x_out, double rtol,
+                    double atol, int64_t itmax, int64_t *id);
+/* Activates uploaded systems and enqueues one step; returns without waiting. */
+int hs_batch_step(hs_batch_t e);
+/* Waits for the last step, retires finished systems (x copied to x_out, their
+ * memory freed) and returns up to `max` results not yet reported. */
+int hs_batch_collect(hs_batch_t e, hs_batch_result *out, int64_t max, int64_t *count);
+/* out[0..9] = active, uploading, free slots, free arena bytes, arena bytes,
+ * largest free arena block, steps, retired, results waiting, tile-kernel grid */
+int hs_batch_info(hs_batch_t e, int64_t *out);
+/* Copies a vector of a resident (submitted, not yet collected) system to host
+ * `out` after the last step: what = 0 x, 1 r, 2 z, 3 b, 4 q (n doubles each),
+ * 5 D_Dl1 (n, widened to FP64), 6 the two partials of each 256-row tile.
+ * For tests and diagnostics. */
+int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out);
+/* Seconds since the engine was created (the clock of hs_batch_result). */
+double hs_batch_clock(hs_batch_t e);
+int hs_batch_destroy(hs_batch_t e);
 
 #ifdef __cplusplus
 }
