@@ -13,7 +13,8 @@ with inputs resident in HBM.  Further keys report the PCG solves (FP64
 smoother and FP32 smoother inside FP64 CG, rtol 1e-8) and the CPU oracle
 baseline timed on this host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-pcg] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-serve] [--no-cpu]
+                    [--serve-only] [--serve-count 4096]
 """
 from __future__ import annotations
 
@@ -107,6 +108,68 @@ def cpu_baseline(A, partsize, budget_s=12.0):
                       f"FP64 packed), oracle restatement with OpenMP, {dt * 1e3:.1f} ms each"}
 
 
+def serving_bench(args, log_fn=log):
+    """Batched persistent PCG serving the stream of SPD systems: N log-uniform in
+    [1e3, 2.6e5] rows, alternating 2D 5-point / 3D 7-point lognormal (sigma 1)
+    grids, FP32 L1GS partsize 32 inside FP64 CG to rtol 1e-8.  The autotuner
+    picks steps and grid on a separate sample; a saturation run (all arrivals
+    at t = 0) measures capacity, then a Poisson stream at 0.8 x capacity gives
+    latency percentiles."""
+    from hybridsmoothers import autotune, problems as pb, serve
+    from hybridsmoothers.batch import BatchSolver
+    N = args.serve_count
+    specs = pb.serving_stream(N, seed=5)
+    log_fn(f"[bench] serving: autotune on a 48-system sample")
+    sample = autotune.sample_systems(pb.serving_stream(48, seed=77))
+    tune = autotune.tune_batch(sample, partsizes=(32,), passes=(4, 8, 16), grids=(0, 1024),
+                               precision="fp32", arena_bytes=32 << 30, log=log_fn)
+    out = {"workload": f"{N} SPD systems, N log-uniform in [1e3, 2.6e5] (2D 5-point / 3D "
+                       f"7-point lognormal sigma=1, alternating), b = U[-1,1]; FP64 CG, FP32 "
+                       f"L1GS partsize 32 symmetric, rtol 1e-8",
+           "autotune": tune}
+    with BatchSolver(partsize=32, precision="fp32", passes=tune["passes"], grid=tune["grid"],
+                     max_systems=args.serve_slots) as eng:
+        log_fn(f"[bench] serving: saturation run ({N} systems at t=0)")
+        rep_sat, _ = serve.serve(eng, specs, serve.poisson_arrivals(N, float("inf")))
+        log_fn(f"[bench] serving: saturation {rep_sat.as_dict()}")
+        lam = 0.8 * rep_sat.solves_per_s
+        log_fn(f"[bench] serving: Poisson arrivals at {lam:.1f}/s")
+        rep, _ = serve.serve(eng, specs, serve.poisson_arrivals(N, lam, seed=7))
+        log_fn(f"[bench] serving: poisson {rep.as_dict()}")
+        out["arena_bytes"] = eng.info()["arena_bytes"]
+    out["saturation"] = rep_sat.as_dict()
+    out["poisson"] = rep.as_dict()
+    out["solves_per_s"] = round(rep.solves_per_s, 2)
+    out["latency_p50_ms"] = round(rep.latency_p50_ms, 3)
+    out["latency_p99_ms"] = round(rep.latency_p99_ms, 3)
+    out["capacity_solves_per_s"] = round(rep_sat.solves_per_s, 2)
+    return out
+
+
+def serving_cpu_baseline(budget_s=10.0):
+    """The oracle's PCG with its L1GS (FP64, partsize 32, OpenMP) on systems of
+    the serving stream, in stream order, until the time budget is spent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    specs = pb.serving_stream(4096, seed=5)
+    threads = min(os.cpu_count() or 1, 16)
+    done, rows, t0 = 0, 0, time.perf_counter()
+    for i, spec in enumerate(specs):
+        A = pb.csr_to_scipy(*pb.stream_system(spec))
+        b = pb.uniform_rhs(A.shape[0], 1000 + i)
+        ref = O.l1gs_build(A, 32, is_sym=True, sweep="symmetric", nthreads=threads)
+        O.pcg(A, b, ref, rtol=1e-8, nthreads=threads)
+        done += 1
+        rows += A.shape[0]
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(done / dt, 3), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} systems of the stream ({rows} rows), oracle L1GS P=32 FP64 "
+                      f"+ CG with OpenMP, build included, {dt:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +180,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline sweep only")
     ap.add_argument("--grid2d", type=int, default=2048)
+    ap.add_argument("--serve-count", type=int, default=4096)
+    ap.add_argument("--serve-slots", type=int, default=1024)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving benchmark")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB per step (counter calibration)")
     args = ap.parse_args()
@@ -140,6 +207,12 @@ def main():
     from hybridsmoothers import problems as pb
 
     hs.init(0)
+    if args.serve_only:
+        res = serving_bench(args)
+        if not args.no_cpu:
+            res["cpu_baseline"] = serving_cpu_baseline()
+        print(json.dumps({"serving": res}), flush=True)
+        return
     g, P = args.grid, args.partsize
     log(f"[bench] building 3D Poisson {g}^3")
     A = pb.poisson3d(g)
@@ -237,6 +310,11 @@ def main():
         for r in pcg:
             r["speedup_vs_fp64_l1gs"] = round(r["solves_per_s"] / base, 3)
         out["pcg_128"] = pcg
+    if not args.quick and not args.no_serve:
+        out["serving"] = serving_bench(args)
+        if not args.no_cpu:
+            log("[bench] serving CPU baseline (oracle)")
+            out["serving"]["cpu_baseline"] = serving_cpu_baseline()
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(A, P)

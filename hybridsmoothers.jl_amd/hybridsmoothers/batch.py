@@ -70,6 +70,27 @@ def csr_arrays(A):
             np.ascontiguousarray(A.data, dtype=np.float64))
 
 
+def iteration_bytes(ptr, partsize: int, precision: str) -> int:
+    """Algorithmic HBM bytes of one fused CG iteration of one system in the
+    engine: the sliced-ELL matrix (8-byte value + 4-byte column per stored
+    entry, slice offsets), z and p_{k-1} gathered once and p_k, q written
+    (32 B/row) in the SpMV; both packed triangles, D_Dl1, r and x read and
+    written, q and p read, z written (56 B/row) in the sweep."""
+    ptr = np.asarray(ptr, dtype=np.int64)
+    n = ptr.size - 1
+    lens = np.diff(ptr)
+    pad = (-n) % 64
+    w = np.concatenate([lens, np.zeros(pad, np.int64)]).reshape(-1, 64).max(axis=1)
+    nent = int(64 * w.sum())
+    tsz = {"fp64": 8, "fp32": 4, "bf16": 2}[precision]
+    dsz = 8 if precision == "fp64" else 4
+    P = partsize
+    nparts = (n + P - 1) // P
+    Pl = n - (nparts - 1) * P
+    tri = (nparts - 1) * P * (P - 1) + Pl * (Pl - 1)
+    return nent * 12 + (w.size + 1) * 8 + tri * tsz + n * (dsz + 88)
+
+
 def default_arena_bytes(fraction: float = 0.5, cap: int = 96 << 30) -> int:
     try:
         import torch

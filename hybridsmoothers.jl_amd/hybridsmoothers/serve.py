@@ -30,7 +30,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import problems as pb
-from .batch import BatchSolver
+from .batch import BatchSolver, iteration_bytes
 
 
 def poisson_arrivals(count: int, rate: float, seed: int = 0) -> np.ndarray:
@@ -107,6 +107,8 @@ class ServeReport:
     all_solved: bool
     rows_total: int
     steps: int
+    algorithmic_bytes: int     # sum over systems of iteration bytes x (iterations + 1)
+    achieved_GBs: float        # algorithmic_bytes / wall time
 
     def as_dict(self):
         return {k: (float(f"{v:.5g}") if isinstance(v, float) else v)
@@ -126,6 +128,7 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
     status = [""] * N
     xs = [None] * N if keep_x else None
     rows = 0
+    ibytes = np.zeros(N, np.int64)
     pending: list[int] = []
     staged: dict[int, tuple] = {}
     nxt = 0
@@ -154,6 +157,8 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
                 del staged[i]
                 pending.pop(k)
                 rows += b.size
+                ibytes[i] = iteration_bytes(A[0] if isinstance(A, tuple) else A.indptr + 1,
+                                            engine.partsize, engine.precision)
             if busy:
                 for s in engine.collect():
                     i = s.tag
@@ -175,6 +180,7 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
     span = wall - float(arrivals[0])
     lat_ms = lat * 1e3
     rate = float(N / arrivals[-1]) if arrivals[-1] > 0 else float("inf")
+    total_bytes = int(np.sum(ibytes * (iters + 1)))  # + 1: the admission pass
     rep = ServeReport(count=N, wall_s=wall, solves_per_s=N / span if span > 0 else float("inf"),
                       latency_p50_ms=float(np.percentile(lat_ms, 50)),
                       latency_p99_ms=float(np.percentile(lat_ms, 99)),
@@ -182,6 +188,7 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
                       rate=rate, iters_mean=float(np.mean(iters)), iters_max=int(np.max(iters)),
                       max_true_relres=float(np.nanmax(relres)),
                       all_solved=all(s == "solved" for s in status), rows_total=int(rows),
-                      steps=engine.info()["steps"] - steps0)
+                      steps=engine.info()["steps"] - steps0, algorithmic_bytes=total_bytes,
+                      achieved_GBs=total_bytes / wall / 1e9)
     records = {"latency_s": lat, "iters": iters, "true_relres": relres, "status": status, "x": xs}
     return rep, records
