@@ -127,14 +127,19 @@ def serving_bench(args, log_fn=log):
                        f"7-point lognormal sigma=1, alternating), b = U[-1,1]; FP64 CG, FP32 "
                        f"L1GS partsize 32 symmetric, rtol 1e-8",
            "autotune": tune}
+    log_fn(f"[bench] serving: generating the {N} requests (clients' matrices, before the clock)")
+    t0 = time.perf_counter()
+    reqs = serve.Prepared(serve.generate(specs, workers=16))
+    out["generation_s"] = round(time.perf_counter() - t0, 3)
     with BatchSolver(partsize=32, precision="fp32", passes=tune["passes"], grid=tune["grid"],
                      max_systems=args.serve_slots) as eng:
         log_fn(f"[bench] serving: saturation run ({N} systems at t=0)")
-        rep_sat, _ = serve.serve(eng, specs, serve.poisson_arrivals(N, float("inf")))
+        rep_sat, _ = serve.serve(eng, specs, serve.poisson_arrivals(N, float("inf")),
+                                 producer=reqs)
         log_fn(f"[bench] serving: saturation {rep_sat.as_dict()}")
         lam = 0.8 * rep_sat.solves_per_s
         log_fn(f"[bench] serving: Poisson arrivals at {lam:.1f}/s")
-        rep, _ = serve.serve(eng, specs, serve.poisson_arrivals(N, lam, seed=7))
+        rep, _ = serve.serve(eng, specs, serve.poisson_arrivals(N, lam, seed=7), producer=reqs)
         log_fn(f"[bench] serving: poisson {rep.as_dict()}")
         out["arena_bytes"] = eng.info()["arena_bytes"]
     out["saturation"] = rep_sat.as_dict()

@@ -47,19 +47,52 @@ constexpr int TILE = 256;  // rows per tile (= one 256-thread workgroup)
 
 enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_RESID = 3, PH_DONE = 4 };
 
-// Descriptor of a resident system (device array, one per slot).
+// Descriptor of a resident system (device array, one per slot): byte offsets
+// of its arrays in the arena.  The kernels add them to the arena base, a
+// kernel argument, so the compiler knows every access is to global memory
+// (pointers loaded from memory would be generic: flat loads, which are also
+// counted against the LDS counter).
 struct Sys {
+    int64_t sptr, col, val, T, D, x, r, z, pa, pb, q, b, part;
+    int64_t n, nparts;
+    int32_t ntiles, pad;
+};
+
+// The same, resolved against the arena base.
+struct SysP {
     const int64_t *sptr;
     const int32_t *col;
     const double *val;
-    const void *T;  // packed triangles (storage type of the engine's precision)
-    const void *D;  // D_Dl1 (FP64, or FP32 for the reduced-precision smoothers)
+    const uint8_t *T;  // packed triangles (storage type of the engine's precision)
+    const uint8_t *D;  // D_Dl1 (FP64, or FP32 for the reduced-precision smoothers)
     double *x, *r, *z, *pa, *pb, *q;
     const double *b;
     double *part;  // two partials per tile
     int64_t n, nparts;
-    int32_t ntiles, pad;
+    int32_t ntiles;
 };
+
+__device__ __forceinline__ SysP load_sys(const Sys *__restrict__ sys, int s, uint8_t *arena) {
+    const Sys d = sys[s];
+    SysP S;
+    S.sptr = reinterpret_cast<const int64_t *>(arena + d.sptr);
+    S.col = reinterpret_cast<const int32_t *>(arena + d.col);
+    S.val = reinterpret_cast<const double *>(arena + d.val);
+    S.T = arena + d.T;
+    S.D = arena + d.D;
+    S.x = reinterpret_cast<double *>(arena + d.x);
+    S.r = reinterpret_cast<double *>(arena + d.r);
+    S.z = reinterpret_cast<double *>(arena + d.z);
+    S.pa = reinterpret_cast<double *>(arena + d.pa);
+    S.pb = reinterpret_cast<double *>(arena + d.pb);
+    S.q = reinterpret_cast<double *>(arena + d.q);
+    S.b = reinterpret_cast<const double *>(arena + d.b);
+    S.part = reinterpret_cast<double *>(arena + d.part);
+    S.n = d.n;
+    S.nparts = d.nparts;
+    S.ntiles = d.ntiles;
+    return S;
+}
 
 struct State {
     CGDev cg;
@@ -84,7 +117,8 @@ __device__ __forceinline__ void tile_range(int nt, int &t0, int &t1) {
 __global__ __launch_bounds__(256) void k_b_spmv(const Tile *__restrict__ tiles,
                                                 const Counts *__restrict__ cnt,
                                                 const Sys *__restrict__ sys,
-                                                const State *__restrict__ st) {
+                                                const State *__restrict__ st,
+                                                uint8_t *__restrict__ arena) {
     __shared__ double sm[4];
     int t0, t1;
     tile_range(cnt->ntiles, t0, t1);
@@ -92,7 +126,7 @@ __global__ __launch_bounds__(256) void k_b_spmv(const Tile *__restrict__ tiles,
         const Tile tl = tiles[t];
         const int ph = st[tl.slot].phase;
         if (ph != PH_ITER && ph != PH_RESID) continue;  // uniform in the workgroup
-        const Sys &S = sys[tl.slot];
+        const SysP S = load_sys(sys, tl.slot, arena);
         const SellView<double> A{S.sptr, S.col, S.val, S.n};
         const int64_t i = (int64_t)tl.tile * TILE + threadIdx.x;
         if (ph == PH_ITER) {
@@ -128,13 +162,13 @@ __global__ __launch_bounds__(256) void k_b_spmv(const Tile *__restrict__ tiles,
 __global__ __launch_bounds__(FIN_BLOCK) void k_b_fin_delta(const int32_t *__restrict__ act,
                                                            const Counts *__restrict__ cnt,
                                                            const Sys *__restrict__ sys,
-                                                           State *st) {
+                                                           State *st, uint8_t *__restrict__ arena) {
     __shared__ double sm[16];
     const int na = cnt->nactive;
     for (int a = blockIdx.x; a < na; a += gridDim.x) {
         const int s = act[a];
         const int ph = st[s].phase;  // read by every thread before the barriers below
-        const Sys &S = sys[s];
+        const SysP S = load_sys(sys, s, arena);
         if (ph == PH_ITER) {
             const double tot = finalize_sum(S.part, (unsigned)S.ntiles, 2, sm);
             if (threadIdx.x == 0 && !cg_scalars_delta(st[s].cg, tot)) st[s].phase = PH_RESID;
@@ -150,41 +184,57 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_b_fin_delta(const int32_t *__rest
     }
 }
 
+// One-triangle layout (MODE_SYMT) when the segments' LDS copies of L take at
+// most 36 KiB per workgroup (four workgroups per CU): partsize 16 in every
+// precision, partsize 32 in FP32 and BF16, partsize 64 in BF16.
+template <int PM, class TS>
+constexpr bool use_symt() {
+    return (TILE / PM) * symt_lds(PM) * (int)sizeof(TS) <= 36 * 1024;
+}
+
 // PM = partsize (16, 32 or 64): 256/PM partitions per tile, one per PM-lane segment.
 template <int PM, class TS, class TC>
 __global__ __launch_bounds__(256) void k_b_sweep(const Tile *__restrict__ tiles,
                                                  const Counts *__restrict__ cnt,
                                                  const Sys *__restrict__ sys,
-                                                 const State *__restrict__ st) {
+                                                 const State *__restrict__ st,
+                                                 uint8_t *__restrict__ arena) {
     __shared__ double sm[4];
     constexpr int SEGS = TILE / PM;
+    constexpr bool SYMT = use_symt<PM, TS>();
+    constexpr int MODE = SYMT ? MODE_SYMT : MODE_SYM;
+    __shared__ TS lds_all[SYMT ? SEGS * symt_lds(PM) : 1];
+    TS *lds = SYMT ? lds_all + (threadIdx.x / PM) * symt_lds(PM) : nullptr;
     int t0, t1;
     tile_range(cnt->ntiles, t0, t1);
     for (int t = t0; t < t1; ++t) {
         const Tile tl = tiles[t];
         const int ph = st[tl.slot].phase;
         if (ph != PH_INIT && ph != PH_ITER) continue;
-        const Sys &S = sys[tl.slot];
+        const SysP S = load_sys(sys, tl.slot, arena);
         const TS *T = reinterpret_cast<const TS *>(S.T);
         const TC *D = reinterpret_cast<const TC *>(S.D);
-        const int lane = threadIdx.x % PM;
+        int lane = threadIdx.x % PM;
+        // Opaque per tile: the sweep's per-lane offsets (dozens per step) are
+        // otherwise hoisted out of the tile loop and held in registers.
+        asm volatile("" : "+v"(lane));
         const int64_t k = (int64_t)tl.tile * SEGS + threadIdx.x / PM;
         double v = 0.0;
-        if (!__all(k >= S.nparts)) {  // whole waves: see sweep_seg_body
-            if (ph == PH_INIT) {
-                // r = b, x = 0, z = M^-1 b; partial b'z = r'z
-                v = sweep_seg_body<PM, MODE_SYM, TS, TC, false>(S.b, S.z, T, D, S.n, PM, S.nparts,
-                                                               k, lane, CGFuse{});
-                const int64_t i = k * PM + lane;
-                if (k < S.nparts && i < S.n) {
-                    S.r[i] = S.b[i];
+        if (!__all(k >= S.nparts)) {  // whole waves: see sweep_seg_run
+            const bool init = (ph == PH_INIT);
+            const CGFuse f{S.r, S.q, S.pa, S.pb, S.x, nullptr, &st[tl.slot].cg};
+            // admission: r = b, x = 0, z = M^-1 b (partial b'z = r'z);
+            // iteration: r -= alpha q, x += alpha p, z = M^-1 r
+            auto input = [&](int64_t i) -> double {
+                if (init) {
+                    const double bi = S.b[i];
+                    S.r[i] = bi;
                     S.x[i] = 0.0;
+                    return bi;
                 }
-            } else {
-                const CGFuse f{S.r, S.q, S.pa, S.pb, S.x, nullptr, &st[tl.slot].cg};
-                v = sweep_seg_body<PM, MODE_SYM, TS, TC, true>(nullptr, S.z, T, D, S.n, PM,
-                                                              S.nparts, k, lane, f);
-            }
+                return cg_prologue(f, i);
+            };
+            v = sweep_seg_run<PM, MODE, TS, TC>(input, S.z, T, D, S.n, PM, S.nparts, k, lane, lds);
         }
         v = block_sum(v, sm);
         if (threadIdx.x == 0) S.part[2 * tl.tile] = v;
@@ -194,14 +244,14 @@ __global__ __launch_bounds__(256) void k_b_sweep(const Tile *__restrict__ tiles,
 __global__ __launch_bounds__(FIN_BLOCK) void k_b_fin_gamma(const int32_t *__restrict__ act,
                                                            const Counts *__restrict__ cnt,
                                                            const Sys *__restrict__ sys,
-                                                           State *st) {
+                                                           State *st, uint8_t *__restrict__ arena) {
     __shared__ double sm[16];
     const int na = cnt->nactive;
     for (int a = blockIdx.x; a < na; a += gridDim.x) {
         const int s = act[a];
         const int ph = st[s].phase;
         if (ph != PH_INIT && ph != PH_ITER) continue;
-        const Sys &S = sys[s];
+        const SysP S = load_sys(sys, s, arena);
         const double tot = finalize_sum(S.part, (unsigned)S.ntiles, 2, sm);
         if (threadIdx.x == 0) {
             const bool done = (ph == PH_INIT) ? cg_scalars_start(st[s].cg, tot)
@@ -218,7 +268,7 @@ template <class TS, class TC>
 __global__ void k_b_setup(int64_t n, int64_t P, const int64_t *__restrict__ sptr,
                           const int32_t *__restrict__ col, const double *__restrict__ val,
                           const int32_t *__restrict__ rlen, double eta, TC *__restrict__ D,
-                          TS *__restrict__ T) {
+                          TS *__restrict__ T, int mode) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const int64_t k = g / P;
@@ -227,7 +277,7 @@ __global__ void k_b_setup(int64_t n, int64_t P, const int64_t *__restrict__ sptr
     const int64_t Pa = e - s + 1;
     const int64_t i = g - s;
     int64_t lb, ub;
-    block_bases(k, P, Pa, MODE_SYM, lb, ub);
+    block_bases(k, P, Pa, mode, lb, ub);
     const int64_t base = sptr[g >> 6] + (g & 63);
     double a = 0.0, d = 0.0;
     const int len = rlen[g];
@@ -242,7 +292,7 @@ __global__ void k_b_setup(int64_t n, int64_t P, const int64_t *__restrict__ sptr
         } else if (j < g) {
             const int64_t jl = j - s;
             T[lb + colL_off(jl, Pa) + (i - jl - 1)] = Scalar<TS>::from_f64(v);
-        } else {
+        } else if (mode == MODE_SYM) {  // the one-triangle layout stores no U
             const int64_t jl = j - s;
             T[ub + colU_off(jl, Pa) + i] = Scalar<TS>::from_f64(v);
         }
@@ -264,7 +314,8 @@ struct Layout {
     int64_t total = 0;
 };
 
-static Layout make_layout(int64_t n, const int64_t *ptr, int64_t P, int tsz, int dsz) {
+static Layout make_layout(int64_t n, const int64_t *ptr, int64_t P, int tsz, int dsz,
+                          bool one_tri) {
     Layout L;
     L.n = n;
     L.nslices = (n + 63) / 64;
@@ -278,6 +329,7 @@ static Layout make_layout(int64_t n, const int64_t *ptr, int64_t P, int tsz, int
     L.ntiles = (n + TILE - 1) / TILE;
     const int64_t Pl = n - (L.nparts - 1) * P;
     L.tri = (L.nparts - 1) * P * (P - 1) + Pl * (Pl - 1);
+    if (one_tri) L.tri /= 2;
     L.o_sptr = 0;
     L.o_col = align256(L.o_sptr + (L.nslices + 1) * 8);
     L.o_val = align256(L.o_col + L.nent * 4);
@@ -406,7 +458,8 @@ struct Slot {
 struct Engine {
     hs_batch_config cfg{};
     int tsz = 8, dsz = 8;
-    int grid = 0;
+    int grid_spmv = 0, grid_sweep = 0;  // workgroups of the two tile kernels
+    bool one_tri = false;               // packed lower triangle only (MODE_SYMT)
     uint8_t *arena_dev = nullptr;
     Arena arena;
     uint8_t *staging = nullptr;
@@ -476,15 +529,50 @@ static void host_alloc(T *&p, size_t count) {
                                hipHostMallocDefault));
 }
 
+template <int PM>
+static bool one_triangle_p(int prec) {
+    if (prec == HS_PREC_FP64) return use_symt<PM, double>();
+    if (prec == HS_PREC_FP32) return use_symt<PM, float>();
+    return use_symt<PM, __hip_bfloat16>();
+}
+
+static bool one_triangle(int P, int prec) {
+    return P == 16 ? one_triangle_p<16>(prec) : (P == 32 ? one_triangle_p<32>(prec)
+                                                         : one_triangle_p<64>(prec));
+}
+
+static int resident_grid(const void *kernel, int cus) {
+    int nb = 0;
+    HS_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, TILE, 0));
+    return std::max(1, nb) * cus;
+}
+
+template <int PM>
+static const void *sweep_fn_p(int prec) {
+    if (prec == HS_PREC_FP64) return reinterpret_cast<const void *>(&k_b_sweep<PM, double, double>);
+    if (prec == HS_PREC_FP32) return reinterpret_cast<const void *>(&k_b_sweep<PM, float, float>);
+    return reinterpret_cast<const void *>(&k_b_sweep<PM, __hip_bfloat16, float>);
+}
+
+static const void *sweep_fn(int P, int prec) {
+    return P == 16 ? sweep_fn_p<16>(prec) : (P == 32 ? sweep_fn_p<32>(prec) : sweep_fn_p<64>(prec));
+}
+
 void Engine::create(const hs_batch_config &c) {
     cfg = c;
     tsz = c.precision == HS_PREC_FP64 ? 8 : (c.precision == HS_PREC_FP32 ? 4 : 2);
     dsz = c.precision == HS_PREC_FP64 ? 8 : 4;
+    one_tri = one_triangle((int)c.partsize, c.precision);
     int dev = 0;
     HS_CHECK_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HS_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
-    grid = c.grid > 0 ? c.grid : 8 * prop.multiProcessorCount;
+    // Persistent tile kernels: by default exactly as many workgroups as fit on
+    // the chip at once (each walks a fixed run of the tile table; a second
+    // round of workgroups would wait for the first to finish).
+    const int cus = prop.multiProcessorCount;
+    grid_spmv = c.grid > 0 ? c.grid : resident_grid(reinterpret_cast<const void *>(&k_b_spmv), cus);
+    grid_sweep = c.grid > 0 ? c.grid : resident_grid(sweep_fn((int)c.partsize, c.precision), cus);
     HS_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HS_CHECK_HIP(hipStreamCreateWithFlags(&us, hipStreamNonBlocking));
     HS_CHECK_HIP(hipEventCreateWithFlags(&ev_step, hipEventDisableTiming));
@@ -525,32 +613,35 @@ void Engine::create(const hs_batch_config &c) {
 
 template <int PM>
 static void launch_sweep(int prec, unsigned grid, hipStream_t s, const Tile *tiles,
-                         const Counts *cnt, const Sys *sys, const State *st) {
+                         const Counts *cnt, const Sys *sys, const State *st, uint8_t *arena) {
     if (prec == HS_PREC_FP64)
         hipLaunchKernelGGL((k_b_sweep<PM, double, double>), dim3(grid), dim3(TILE), 0, s, tiles,
-                           cnt, sys, st);
+                           cnt, sys, st, arena);
     else if (prec == HS_PREC_FP32)
         hipLaunchKernelGGL((k_b_sweep<PM, float, float>), dim3(grid), dim3(TILE), 0, s, tiles, cnt,
-                           sys, st);
+                           sys, st, arena);
     else
         hipLaunchKernelGGL((k_b_sweep<PM, __hip_bfloat16, float>), dim3(grid), dim3(TILE), 0, s,
-                           tiles, cnt, sys, st);
+                           tiles, cnt, sys, st, arena);
 }
 
 void Engine::enqueue_pass(hipStream_t s) {
     const unsigned fin_grid = (unsigned)std::min<int64_t>(cfg.max_systems, 256);
-    hipLaunchKernelGGL(k_b_spmv, dim3(grid), dim3(TILE), 0, s, d_tiles.get(), d_cnt.get(),
-                       d_sys.get(), d_st.get());
-    hipLaunchKernelGGL(k_b_fin_delta, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(),
-                       d_cnt.get(), d_sys.get(), d_st.get());
+    const Tile *tl = d_tiles.get();
+    const Counts *cn = d_cnt.get();
+    const Sys *sy = d_sys.get();
+    State *stt = d_st.get();
+    hipLaunchKernelGGL(k_b_spmv, dim3(grid_spmv), dim3(TILE), 0, s, tl, cn, sy, stt, arena_dev);
+    hipLaunchKernelGGL(k_b_fin_delta, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(), cn, sy,
+                       stt, arena_dev);
     if (cfg.partsize == 16)
-        launch_sweep<16>(cfg.precision, grid, s, d_tiles.get(), d_cnt.get(), d_sys.get(), d_st.get());
+        launch_sweep<16>(cfg.precision, grid_sweep, s, tl, cn, sy, stt, arena_dev);
     else if (cfg.partsize == 32)
-        launch_sweep<32>(cfg.precision, grid, s, d_tiles.get(), d_cnt.get(), d_sys.get(), d_st.get());
+        launch_sweep<32>(cfg.precision, grid_sweep, s, tl, cn, sy, stt, arena_dev);
     else
-        launch_sweep<64>(cfg.precision, grid, s, d_tiles.get(), d_cnt.get(), d_sys.get(), d_st.get());
-    hipLaunchKernelGGL(k_b_fin_gamma, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(),
-                       d_cnt.get(), d_sys.get(), d_st.get());
+        launch_sweep<64>(cfg.precision, grid_sweep, s, tl, cn, sy, stt, arena_dev);
+    hipLaunchKernelGGL(k_b_fin_gamma, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(), cn, sy,
+                       stt, arena_dev);
     check_launch();
 }
 
@@ -570,7 +661,7 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
     HS_REQUIRE(ptr[0] == 1, HS_EINVAL, "pointer array must be 1-based (ptr[0] == 1)");
     for (int64_t i = 0; i < n; ++i)
         HS_REQUIRE(ptr[i + 1] >= ptr[i], HS_EINVAL, "pointer array must be non-decreasing");
-    const Layout L = make_layout(n, ptr, cfg.partsize, tsz, dsz);
+    const Layout L = make_layout(n, ptr, cfg.partsize, tsz, dsz, one_tri);
     HS_REQUIRE(L.upload <= cfg.staging_bytes, HS_EINVAL,
                "system larger than the engine's upload staging buffer");
     HS_REQUIRE(L.total <= cfg.arena_bytes, HS_EINVAL, "system larger than the engine's arena");
@@ -590,6 +681,7 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
         HS_CHECK_HIP(hipMemcpyAsync(blk, src, L.upload, hipMemcpyHostToDevice, us));
         if (L.tri) HS_CHECK_HIP(hipMemsetAsync(blk + L.o_T, 0, L.tri * tsz, us));
         const dim3 g(grid_for(n, 256)), t(256);
+        const int mode = one_tri ? MODE_SYMT : MODE_SYM;
         const auto *sp = reinterpret_cast<const int64_t *>(blk + L.o_sptr);
         const auto *cl = reinterpret_cast<const int32_t *>(blk + L.o_col);
         const auto *vl = reinterpret_cast<const double *>(blk + L.o_val);
@@ -597,15 +689,15 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
         if (cfg.precision == HS_PREC_FP64)
             hipLaunchKernelGGL((k_b_setup<double, double>), g, t, 0, us, n, cfg.partsize, sp, cl,
                                vl, rl, cfg.eta, reinterpret_cast<double *>(blk + L.o_D),
-                               reinterpret_cast<double *>(blk + L.o_T));
+                               reinterpret_cast<double *>(blk + L.o_T), mode);
         else if (cfg.precision == HS_PREC_FP32)
             hipLaunchKernelGGL((k_b_setup<float, float>), g, t, 0, us, n, cfg.partsize, sp, cl, vl,
                                rl, cfg.eta, reinterpret_cast<float *>(blk + L.o_D),
-                               reinterpret_cast<float *>(blk + L.o_T));
+                               reinterpret_cast<float *>(blk + L.o_T), mode);
         else
             hipLaunchKernelGGL((k_b_setup<__hip_bfloat16, float>), g, t, 0, us, n, cfg.partsize,
                                sp, cl, vl, rl, cfg.eta, reinterpret_cast<float *>(blk + L.o_D),
-                               reinterpret_cast<__hip_bfloat16 *>(blk + L.o_T));
+                               reinterpret_cast<__hip_bfloat16 *>(blk + L.o_T), mode);
         check_launch();
         HS_CHECK_HIP(hipEventRecord(slots[s].ready, us));
     } catch (...) {
@@ -673,21 +765,21 @@ void Engine::step() {
         for (int s : activated) {
             const Slot &S = slots[s];
             const Layout &L = S.lay;
-            uint8_t *blk = arena_dev + S.off;
+            const int64_t o = S.off;
             Sys &d = h_sys[s];
-            d.sptr = reinterpret_cast<const int64_t *>(blk + L.o_sptr);
-            d.col = reinterpret_cast<const int32_t *>(blk + L.o_col);
-            d.val = reinterpret_cast<const double *>(blk + L.o_val);
-            d.T = blk + L.o_T;
-            d.D = blk + L.o_D;
-            d.x = reinterpret_cast<double *>(blk + L.o_x);
-            d.r = reinterpret_cast<double *>(blk + L.o_r);
-            d.z = reinterpret_cast<double *>(blk + L.o_z);
-            d.pa = reinterpret_cast<double *>(blk + L.o_pa);
-            d.pb = reinterpret_cast<double *>(blk + L.o_pb);
-            d.q = reinterpret_cast<double *>(blk + L.o_q);
-            d.b = reinterpret_cast<const double *>(blk + L.o_b);
-            d.part = reinterpret_cast<double *>(blk + L.o_part);
+            d.sptr = o + L.o_sptr;
+            d.col = o + L.o_col;
+            d.val = o + L.o_val;
+            d.T = o + L.o_T;
+            d.D = o + L.o_D;
+            d.x = o + L.o_x;
+            d.r = o + L.o_r;
+            d.z = o + L.o_z;
+            d.pa = o + L.o_pa;
+            d.pb = o + L.o_pb;
+            d.q = o + L.o_q;
+            d.b = o + L.o_b;
+            d.part = o + L.o_part;
             d.n = L.n;
             d.nparts = L.nparts;
             d.ntiles = (int32_t)L.ntiles;
@@ -794,7 +886,7 @@ int hs_batch_footprint(hs_batch_t e, int64_t n, const int64_t *ptr, int64_t *byt
     return guard([&] {
         HS_REQUIRE(e && ptr && bytes, HS_EINVAL, "NULL argument");
         HS_REQUIRE(n > 0, HS_EINVAL, "system size must be positive");
-        *bytes = batch::make_layout(n, ptr, e->cfg.partsize, e->tsz, e->dsz).total;
+        *bytes = batch::make_layout(n, ptr, e->cfg.partsize, e->tsz, e->dsz, e->one_tri).total;
     });
 }
 
@@ -847,7 +939,9 @@ int hs_batch_info(hs_batch_t e, int64_t *out) {
         out[6] = e->steps;
         out[7] = e->retired;
         out[8] = (int64_t)e->results.size();
-        out[9] = e->grid;
+        out[9] = e->grid_spmv;
+        out[10] = e->grid_sweep;
+        out[11] = e->one_tri ? 1 : 2;  // packed triangles per partition
     });
 }
 

@@ -68,19 +68,42 @@ __device__ __forceinline__ bool cg_scalars_delta(CGDev &st, double delta) {
 // Row i of the fused SpMV of CG iteration k: p_k = z + beta p_{k-1} (p_0 = z)
 // formed on the fly for row i and its columns, q_i = (A p_k)_i in row order.
 // Stores p_k[i] and q[i]; returns p_k[i] * q_i (the p'q partial of the row).
+// The row's entries are taken in batches of 8: all column indices and values
+// of a batch are loaded before any gather (indices past the row's slice width
+// re-read its last entry), so a wave keeps its loads in flight instead of one
+// dependent index -> value chain per entry; the products are still added one
+// by one in row order.
 __device__ __forceinline__ double cg_spmv_row(const SellView<double> &A,
                                               const double *__restrict__ z,
                                               const double *__restrict__ pold,
                                               double *__restrict__ pnew, double *__restrict__ q,
                                               bool first, double beta, int64_t i) {
+    constexpr int B = 8;
     const int64_t s = i >> 6;
-    const int l = (int)(i & 63);
-    const int64_t e = A.sptr[s + 1];
+    const int64_t p0 = A.sptr[s] + (i & 63);
+    const int w = (int)((A.sptr[s + 1] - A.sptr[s]) >> 6);  // slice width, same in the wave
     double acc = 0.0;
-    for (int64_t p = A.sptr[s] + l; p < e; p += 64) {
-        const int32_t c = A.col[p];
-        const double pc = first ? z[c] : z[c] + beta * pold[c];
-        acc = acc + A.val[p] * pc;
+    for (int k0 = 0; k0 < w; k0 += B) {
+        int32_t c[B];
+        double v[B], zc[B], pc[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int64_t p = p0 + 64 * (int64_t)min(k0 + k, w - 1);
+            c[k] = A.col[p];
+            v[k] = A.val[p];
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            zc[k] = z[c[k]];
+            pc[k] = first ? 0.0 : pold[c[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            if (k0 + k < w) {
+                const double pv = first ? zc[k] : zc[k] + beta * pc[k];
+                acc = acc + v[k] * pv;
+            }
+        }
     }
     const double pi = first ? z[i] : z[i] + beta * pold[i];
     pnew[i] = pi;

@@ -13,7 +13,9 @@
 
 namespace hs {
 
-enum { MODE_FWD = 0, MODE_BWD = 1, MODE_SYM = 2 };
+// MODE_SYMT: symmetric sweep of a symmetric matrix from the lower triangle
+// alone (inside a partition U = L^T), stored like a forward sweep's buffer.
+enum { MODE_FWD = 0, MODE_BWD = 1, MODE_SYM = 2, MODE_SYMT = 3 };
 
 // ---------------------------------------------------------------------------
 // layout helpers
@@ -28,6 +30,9 @@ __host__ __device__ __forceinline__ int64_t colU_off(int64_t j, int64_t Pa) {
 __host__ __device__ __forceinline__ int64_t blk_stride(int64_t P, int mode) {
     return mode == MODE_SYM ? P * (P - 1) : (P * (P - 1)) / 2;
 }
+// elements of a segment's LDS copy of L in the one-triangle symmetric sweep:
+// PM rows of PM + 1 (see sweep_full_symt)
+__host__ __device__ constexpr int symt_lds(int PM) { return PM * (PM + 1); }
 // start of the lower and upper triangles of partition k (Pa rows)
 __host__ __device__ __forceinline__ void block_bases(int64_t k, int64_t P, int64_t Pa, int mode,
                                                      int64_t &lb, int64_t &ub) {
@@ -182,7 +187,7 @@ __device__ __forceinline__ TC sweep_generic(TC xr, TC dr, TC rr, int lane, int P
         }
         yr = yfin;
     }
-    if constexpr (MODE == MODE_SYM) xr = dr * yr;  // y .= D_Dl1 .* y  (:897)
+    if constexpr (MODE == MODE_SYM || MODE == MODE_SYMT) xr = dr * yr;  // y .= D_Dl1 .* y (:897)
     if constexpr (MODE != MODE_FWD) {
         TC acc = (TC)0, yfin = (TC)0;
 #pragma unroll
@@ -193,7 +198,13 @@ __device__ __forceinline__ TC sweep_generic(TC xr, TC dr, TC rr, int lane, int P
                 yfin = (lane == j) ? q : yfin;
                 if (j > 0) {
                     TC yj = seg_bcast<PM>(q, j);
-                    TC u = (lane < j) ? load_as<TC>(Uk[colU_off(j, Pa) + lane]) : (TC)0;
+                    TC u = (TC)0;
+                    if (lane < j) {
+                        // U(i, j) = L(j, i): row j of column i of the lower triangle
+                        u = (MODE == MODE_SYMT)
+                                ? load_as<TC>(Lk[colL_off(lane, Pa) + (j - lane - 1)])
+                                : load_as<TC>(Uk[colU_off(j, Pa) + lane]);
+                    }
                     if (lane < j) acc = acc + u * yj;
                 }
             }
@@ -270,6 +281,70 @@ __device__ __forceinline__ TC sweep_full(TC xr, TC dr, TC rr, int lane,
 }
 
 
+// Full partitions of the one-triangle symmetric sweep.  The forward pass is
+// the streamed right-looking recurrence over the columns of L; each lane also
+// stores the value it loads into its own row of a copy of L in the segment's
+// LDS area: lane i writes L(i, j) at i*(PM+1) + j at step j.  Lanes i <= j
+// hold a dead value and write it into their row's unused upper part, so the
+// store needs no lane mask; the row stride PM + 1 spreads a step's stores
+// over distinct banks.  The backward pass needs U(i, j) = L(j, i) in lane i
+// at step j: row j of the copy, j*(PM+1) + i, one contiguous read, prefetched
+// a few steps ahead.  The products and their order are those of
+// sweep_full<PM, MODE_SYM>, so the results are bit-identical to the
+// two-triangle sweep of the same (symmetric) matrix.
+template <int PM, int RING, class TS, class TC>
+__device__ __forceinline__ TC sweep_full_symt(TC xr, TC dr, TC rr, int lane,
+                                              const TS *__restrict__ Lk, TS *lds) {
+    constexpr int NL = PM - 1;
+    constexpr int R = RING < NL ? RING : NL;
+    constexpr int RS = PM + 1;  // row stride of the copy
+    auto load_col = [&](int j) -> TS {
+        const int off = (j == 0) ? (lane > 0 ? lane - 1 : 0) : lane - j - 1;
+        return Lk[colL_off(j, PM) + off];
+    };
+    TS *const wrow = lds + lane * RS;  // this lane's row
+    const TS *const rcol = lds + lane;  // this lane's column of every row
+    TS ring[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) ring[c] = load_col(c);
+    TC acc = (TC)0, yfin = (TC)0;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+        // keep each step's load, LDS store and arithmetic together: hoisted
+        // stores would wait for their loads long before the step needs them
+        __builtin_amdgcn_sched_barrier(0);
+        const TS v = ring[j % R];
+        if (j + R < NL) ring[j % R] = load_col(j + R);
+        wrow[j] = v;
+        TC q = quot_fast(xr - acc, dr, rr);
+        TC yj = bcast_keep<PM>(q, j, yfin);
+        acc = acc + load_as<TC>(v) * yj;  // dead for lanes <= j
+    }
+    TC q = quot_fast(xr - acc, dr, rr);
+    TC yr = (lane == PM - 1) ? q : yfin;
+    const TC xb = dr * yr;  // y .= D_Dl1 .* y  (:897)
+    // the copy is complete; other lanes' rows are read below, RB steps ahead
+    // (LDS latency is short: a few rows in flight hide it)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    constexpr int RB = 4 < NL ? 4 : NL;
+    TS rb[RB];
+#pragma unroll
+    for (int c = 0; c < RB; ++c) rb[c] = rcol[(PM - 1 - c) * RS];
+    acc = (TC)0;
+    yfin = (TC)0;
+#pragma unroll
+    for (int jj = 0; jj < NL; ++jj) {
+        const int j = PM - 1 - jj;
+        const TS v = rb[jj % RB];
+        if (jj + RB < NL) rb[jj % RB] = rcol[(PM - 1 - (jj + RB)) * RS];
+        TC qb = quot_fast(xb - acc, dr, rr);
+        TC yj = bcast_keep<PM>(qb, j, yfin);
+        acc = acc + load_as<TC>(v) * yj;  // dead for lanes >= j
+    }
+    q = quot_fast(xb - acc, dr, rr);
+    return (lane == 0) ? q : yfin;
+}
+
 // Row prologue of the fused CG form: r -= alpha q; x += alpha p (the k_cg_axpy
 // arithmetic); returns the updated r, the sweep's input.
 __device__ __forceinline__ double cg_prologue(const CGFuse &f, int64_t i) {
@@ -282,19 +357,20 @@ __device__ __forceinline__ double cg_prologue(const CGFuse &f, int64_t i) {
 }
 
 
-// One partition per PM-lane segment; returns r_i * z_i of the lane's row
-// (fused CG form, or x_i * y_i) or 0.  Every lane of the wave must call it
-// together: the full-partition recurrence moves values between the segments
-// of a wave (v_readlane / v_writelane, which ignore the execution mask), so
-// the choice of recurrence is made once per wave, and a segment past the last
-// partition (k >= nparts) runs the last partition's recurrence on dead lanes
-// -- its loads stay inside the buffers and it stores nothing.
-template <int PM, int MODE, class TS, class TC, bool CG>
-__device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
-                                                 const TS *__restrict__ T,
-                                                 const TC *__restrict__ D, int64_t n, int P,
-                                                 int64_t nparts, int64_t k, int lane,
-                                                 const CGFuse &f) {
+// One partition per PM-lane segment; returns x_i * y_i of the lane's row
+// (r_i * z_i in the CG forms) or 0.  `input(i)` supplies the sweep's input
+// for row i (and may update vectors of that row).  Every lane of the wave
+// must call it together: the full-partition recurrence moves values between
+// the segments of a wave (v_readlane / v_writelane, which ignore the
+// execution mask), so the choice of recurrence is made once per wave, and a
+// segment past the last partition (k >= nparts) runs the last partition's
+// recurrence on dead lanes -- its loads stay inside the buffers and it stores
+// nothing.  MODE_SYMT takes `lds`: this segment's symt_lds(PM) elements.
+template <int PM, int MODE, class TS, class TC, class In>
+__device__ __forceinline__ double sweep_seg_run(In &&input, double *y, const TS *__restrict__ T,
+                                                const TC *__restrict__ D, int64_t n, int P,
+                                                int64_t nparts, int64_t k, int lane,
+                                                TS *lds = nullptr) {
     const bool valid = k < nparts;
     const int64_t kk = valid ? k : nparts - 1;
     const int64_t s = kk * (int64_t)P;
@@ -307,7 +383,7 @@ __device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
     const TS *Uk = T + ub;
 
     double xin = 0.0;
-    if (live) xin = CG ? cg_prologue(f, s + lane) : x[s + lane];
+    if (live) xin = input(s + lane);
     const TC xr = (TC)xin;
     const TC dr = live ? D[s + lane] : (TC)1;
     const TC rr = (TC)1 / dr;
@@ -315,12 +391,31 @@ __device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
     if (__any(!safe_divisor(dr))) {
         yr = sweep_generic<PM, MODE, false, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
     } else if (__all(Pa == PM)) {
-        yr = sweep_full<PM, MODE, 16, TS, TC>(xr, dr, rr, lane, Lk, Uk);
+        if constexpr (MODE == MODE_SYMT)
+            yr = sweep_full_symt<PM, 16, TS, TC>(xr, dr, rr, lane, Lk, lds);
+        else
+            yr = sweep_full<PM, MODE, 16, TS, TC>(xr, dr, rr, lane, Lk, Uk);
     } else {
         yr = sweep_generic<PM, MODE, true, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
     }
     if (live) y[s + lane] = (double)yr;
     return live ? xin * (double)yr : 0.0;
+}
+
+// The plain form (input x) and the fused CG form (input r - alpha q, with the
+// CG vector updates of cg_prologue).
+template <int PM, int MODE, class TS, class TC, bool CG>
+__device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
+                                                 const TS *__restrict__ T,
+                                                 const TC *__restrict__ D, int64_t n, int P,
+                                                 int64_t nparts, int64_t k, int lane,
+                                                 const CGFuse &f, TS *lds = nullptr) {
+    if constexpr (CG)
+        return sweep_seg_run<PM, MODE, TS, TC>([&](int64_t i) { return cg_prologue(f, i); }, y, T,
+                                               D, n, P, nparts, k, lane, lds);
+    else
+        return sweep_seg_run<PM, MODE, TS, TC>([&](int64_t i) { return x[i]; }, y, T, D, n, P,
+                                               nparts, k, lane, lds);
 }
 
 }  // namespace hs

@@ -70,12 +70,13 @@ def csr_arrays(A):
             np.ascontiguousarray(A.data, dtype=np.float64))
 
 
-def iteration_bytes(ptr, partsize: int, precision: str) -> int:
+def iteration_bytes(ptr, partsize: int, precision: str, triangles: int = 2) -> int:
     """Algorithmic HBM bytes of one fused CG iteration of one system in the
     engine: the sliced-ELL matrix (8-byte value + 4-byte column per stored
     entry, slice offsets), z and p_{k-1} gathered once and p_k, q written
-    (32 B/row) in the SpMV; both packed triangles, D_Dl1, r and x read and
-    written, q and p read, z written (56 B/row) in the sweep."""
+    (32 B/row) in the SpMV; the packed triangle(s) as stored (`triangles` =
+    1: L only, U = L^T), D_Dl1, r and x read and written, q and p read, z
+    written (56 B/row) in the sweep."""
     ptr = np.asarray(ptr, dtype=np.int64)
     n = ptr.size - 1
     lens = np.diff(ptr)
@@ -87,7 +88,7 @@ def iteration_bytes(ptr, partsize: int, precision: str) -> int:
     P = partsize
     nparts = (n + P - 1) // P
     Pl = n - (nparts - 1) * P
-    tri = (nparts - 1) * P * (P - 1) + Pl * (Pl - 1)
+    tri = ((nparts - 1) * P * (P - 1) + Pl * (Pl - 1)) * triangles // 2
     return nent * 12 + (w.size + 1) * 8 + tri * tsz + n * (dsz + 88)
 
 
@@ -115,6 +116,7 @@ class BatchSolver:
         self.partsize, self.precision, self.passes = int(partsize), precision, int(passes)
         self._h = C.c_void_p()
         L.check(L.lib.hs_batch_create(C.byref(cfg), C.byref(self._h)))
+        self.triangles = self.info()["triangles"]
         self._live: dict[int, tuple[np.ndarray, Any]] = {}  # id -> (x buffer, tag)
         self._res = (L.BatchResult * 256)()
 
@@ -167,10 +169,11 @@ class BatchSolver:
         return len(self._live)
 
     def info(self) -> dict:
-        v = np.zeros(10, np.int64)
+        v = np.zeros(12, np.int64)
         L.check(L.lib.hs_batch_info(self._h, v.ctypes.data))
         keys = ("active", "uploading", "free_slots", "free_bytes", "arena_bytes",
-                "largest_free_block", "steps", "retired", "results_waiting", "grid")
+                "largest_free_block", "steps", "retired", "results_waiting", "grid_spmv",
+                "grid_sweep", "triangles")
         return dict(zip(keys, (int(t) for t in v)))
 
     _PEEK = {"x": 0, "r": 1, "z": 2, "b": 3, "q": 4, "D": 5, "partials": 6}

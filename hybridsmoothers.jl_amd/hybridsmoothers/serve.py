@@ -115,10 +115,39 @@ class ServeReport:
                 for k, v in self.__dict__.items()}
 
 
+class Prepared:
+    """Requests whose matrices already exist (the clients built them before
+    sending): same interface as Producer.get."""
+
+    def __init__(self, systems):
+        self.systems = systems
+
+    def get(self, i: int):
+        return self.systems[i]
+
+    def close(self):
+        pass
+
+
+def generate(specs, rhs_seed: int = 1000, workers: int = 8):
+    """All systems of a stream, generated in parallel: [(CSR arrays, b), ...]."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
+        A = pb.stream_system(specs[i])
+        return A, pb.uniform_rhs(A[0].size - 1, rhs_seed + i)
+
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(one, range(len(specs))))
+
+
 def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
-          producer: Producer | None = None, window: int = 16, keep_x: bool = False,
-          on_result=None):
-    """Runs the stream through `engine`; returns (ServeReport, per-request records)."""
+          producer=None, window: int = 16, keep_x: bool = False, on_result=None):
+    """Runs the stream through `engine`; returns (ServeReport, per-request records).
+
+    `producer` supplies request i's (matrix, b) through get(i): a Producer
+    (generation on worker threads, overlapping the loop; its time counts in
+    the latencies) or Prepared (pre-built requests).  Default: a Producer."""
     N = len(specs)
     own = producer is None
     prod = producer or Producer(specs)
@@ -158,7 +187,7 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
                 pending.pop(k)
                 rows += b.size
                 ibytes[i] = iteration_bytes(A[0] if isinstance(A, tuple) else A.indptr + 1,
-                                            engine.partsize, engine.precision)
+                                            engine.partsize, engine.precision, engine.triangles)
             if busy:
                 for s in engine.collect():
                     i = s.tag

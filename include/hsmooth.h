@@ -189,7 +189,7 @@ typedef struct {
     int64_t partsize;       /* 16, 32 or 64 */
     int precision;          /* smoother storage: HS_PREC_FP64, HS_PREC_FP32 or HS_PREC_BF16 */
     int passes;             /* CG iterations per step (one graph launch) */
-    int grid;               /* workgroups of the tile kernels; 0 = 8 per CU */
+    int grid;               /* workgroups of the tile kernels; 0 = as many as are resident at once */
     double eta;             /* the l1 rule's eta (reference default 1.5) */
 } hs_batch_config;
 
@@ -222,8 +222,11 @@ int hs_batch_step(hs_batch_t e);
 /* Waits for the last step, retires finished systems (x copied to x_out, their
  * memory freed) and returns up to `max` results not yet reported. */
 int hs_batch_collect(hs_batch_t e, hs_batch_result *out, int64_t max, int64_t *count);
-/* out[0..9] = active, uploading, free slots, free arena bytes, arena bytes,
- * largest free arena block, steps, retired, results waiting, tile-kernel grid */
+/* out[0..11] = active, uploading, free slots, free arena bytes, arena bytes,
+ * largest free arena block, steps, retired, results waiting, the workgroups
+ * of the SpMV and sweep tile kernels, and the packed triangles stored per
+ * partition (1: the lower one only -- U = L^T -- whenever the sweep's LDS copy
+ * of L fits: partsize 16, partsize 32 in FP32/BF16, partsize 64 in BF16; 2: both) */
 int hs_batch_info(hs_batch_t e, int64_t *out);
 /* Copies a vector of a resident (submitted, not yet collected) system to host
  * `out` after the last step: what = 0 x, 1 r, 2 z, 3 b, 4 q (n doubles each),
