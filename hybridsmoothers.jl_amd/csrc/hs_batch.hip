@@ -21,9 +21,9 @@
 //   k_b_sweep     tiles: r -= alpha q, x += alpha p, z = M^-1 r, r'z partial
 //                 (a newly admitted system: r = b, x = 0, z = M^-1 b)
 //   k_b_fin_gamma systems: the CG scalars and the stopping test
-// The tile kernels are persistent: a fixed grid, each workgroup walking a
-// contiguous run of the tile table (neighbouring tiles of a system share the
-// workgroup's XCD and its L2).  A fixed number of passes is captured once in
+// The sweep is persistent: a fixed grid, each workgroup walking a contiguous
+// run of the tile table (neighbouring tiles of a system share the workgroup's
+// XCD and its L2); the SpMV takes one tile per workgroup of a large grid.  A fixed number of passes is captured once in
 // a HIP graph; admission and retirement only rewrite the device tables
 // between graph launches, so the graph is never re-captured.
 //
@@ -43,7 +43,8 @@
 namespace hs {
 namespace batch {
 
-constexpr int TILE = 256;  // rows per tile (= one 256-thread workgroup)
+constexpr int TILE = 256;         // rows per tile (= one 256-thread workgroup)
+constexpr int SPMV_GRID = 65536;  // workgroups of the SpMV launch (16.8 M rows per pass)
 
 enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_RESID = 3, PH_DONE = 4 };
 
@@ -114,28 +115,44 @@ __device__ __forceinline__ void tile_range(int nt, int &t0, int &t1) {
     t1 = min(t0 + per, nt);
 }
 
+// A workgroup's run of tiles mostly belongs to one or a few systems, in tile
+// order: the system's descriptor and scalars are loaded when the slot changes
+// (not per tile), and the next tile entry is fetched one tile ahead, so a
+// tile starts its vector loads without a chain of dependent scalar loads.
+
+// The SpMV runs one tile per workgroup over a large fixed grid (workgroups
+// past the tile count exit at once; a grid smaller than the tile count
+// strides): the hardware dispatcher then balances the tiles, which measured
+// faster than persistent runs for this short, load-bound kernel.
 __global__ __launch_bounds__(256) void k_b_spmv(const Tile *__restrict__ tiles,
                                                 const Counts *__restrict__ cnt,
                                                 const Sys *__restrict__ sys,
                                                 const State *__restrict__ st,
                                                 uint8_t *__restrict__ arena) {
     __shared__ double sm[4];
-    int t0, t1;
-    tile_range(cnt->ntiles, t0, t1);
-    for (int t = t0; t < t1; ++t) {
+    const int nt = cnt->ntiles;
+    int cur = -1, ph = PH_EMPTY;
+    SysP S{};
+    bool first = false;
+    double beta = 0.0;
+    const double *pold = nullptr;
+    double *pnew = nullptr;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile tl = tiles[t];
-        const int ph = st[tl.slot].phase;
+        if (tl.slot != cur) {
+            cur = tl.slot;
+            ph = st[cur].phase;
+            S = load_sys(sys, cur, arena);
+            const long long k = st[cur].cg.iter;
+            first = (k == 0);
+            beta = first ? 0.0 : st[cur].cg.beta;
+            pold = (k & 1) ? S.pa : S.pb;
+            pnew = (k & 1) ? S.pb : S.pa;
+        }
         if (ph != PH_ITER && ph != PH_RESID) continue;  // uniform in the workgroup
-        const SysP S = load_sys(sys, tl.slot, arena);
         const SellView<double> A{S.sptr, S.col, S.val, S.n};
         const int64_t i = (int64_t)tl.tile * TILE + threadIdx.x;
         if (ph == PH_ITER) {
-            const CGDev &B = st[tl.slot].cg;
-            const long long k = B.iter;
-            const bool first = (k == 0);
-            const double beta = first ? 0.0 : B.beta;
-            const double *pold = (k & 1) ? S.pa : S.pb;
-            double *pnew = (k & 1) ? S.pb : S.pa;
             double d = 0.0;
             if (i < S.n) d = cg_spmv_row(A, S.z, pold, pnew, S.q, first, beta, i);
             d = block_sum(d, sm);
@@ -184,16 +201,18 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_b_fin_delta(const int32_t *__rest
     }
 }
 
-// One-triangle layout (MODE_SYMT) when the segments' LDS copies of L take at
-// most 36 KiB per workgroup (four workgroups per CU): partsize 16 in every
-// precision, partsize 32 in FP32 and BF16, partsize 64 in BF16.
+// The one-triangle layout (MODE_SYMT) is available when the segments' LDS
+// copies of L take at most 36 KiB per workgroup (four workgroups per CU):
+// partsize 16 in every precision, partsize 32 in FP32 and BF16, partsize 64
+// in BF16.  It halves the triangle bytes; the two-triangle kernel needs no
+// LDS and keeps more waves resident.
 template <int PM, class TS>
-constexpr bool use_symt() {
+constexpr bool symt_fits() {
     return (TILE / PM) * symt_lds(PM) * (int)sizeof(TS) <= 36 * 1024;
 }
 
 // PM = partsize (16, 32 or 64): 256/PM partitions per tile, one per PM-lane segment.
-template <int PM, class TS, class TC>
+template <int PM, class TS, class TC, bool SYMT>
 __global__ __launch_bounds__(256) void k_b_sweep(const Tile *__restrict__ tiles,
                                                  const Counts *__restrict__ cnt,
                                                  const Sys *__restrict__ sys,
@@ -201,17 +220,29 @@ __global__ __launch_bounds__(256) void k_b_sweep(const Tile *__restrict__ tiles,
                                                  uint8_t *__restrict__ arena) {
     __shared__ double sm[4];
     constexpr int SEGS = TILE / PM;
-    constexpr bool SYMT = use_symt<PM, TS>();
     constexpr int MODE = SYMT ? MODE_SYMT : MODE_SYM;
     __shared__ TS lds_all[SYMT ? SEGS * symt_lds(PM) : 1];
     TS *lds = SYMT ? lds_all + (threadIdx.x / PM) * symt_lds(PM) : nullptr;
     int t0, t1;
     tile_range(cnt->ntiles, t0, t1);
+    if (t0 >= t1) return;
+    Tile nxt = tiles[t0];
+    int cur = -1, ph = PH_EMPTY;
+    SysP S{};
+    double alpha = 0.0;
+    const double *p = nullptr;
     for (int t = t0; t < t1; ++t) {
-        const Tile tl = tiles[t];
-        const int ph = st[tl.slot].phase;
+        const Tile tl = nxt;
+        if (t + 1 < t1) nxt = tiles[t + 1];
+        if (tl.slot != cur) {
+            cur = tl.slot;
+            ph = st[cur].phase;
+            S = load_sys(sys, cur, arena);
+            // the alpha and p of cg_prologue, once per system
+            alpha = st[cur].cg.gamma / st[cur].cg.delta;
+            p = (st[cur].cg.iter & 1) ? S.pb : S.pa;
+        }
         if (ph != PH_INIT && ph != PH_ITER) continue;
-        const SysP S = load_sys(sys, tl.slot, arena);
         const TS *T = reinterpret_cast<const TS *>(S.T);
         const TC *D = reinterpret_cast<const TC *>(S.D);
         int lane = threadIdx.x % PM;
@@ -222,9 +253,8 @@ __global__ __launch_bounds__(256) void k_b_sweep(const Tile *__restrict__ tiles,
         double v = 0.0;
         if (!__all(k >= S.nparts)) {  // whole waves: see sweep_seg_run
             const bool init = (ph == PH_INIT);
-            const CGFuse f{S.r, S.q, S.pa, S.pb, S.x, nullptr, &st[tl.slot].cg};
             // admission: r = b, x = 0, z = M^-1 b (partial b'z = r'z);
-            // iteration: r -= alpha q, x += alpha p, z = M^-1 r
+            // iteration: r -= alpha q, x += alpha p, z = M^-1 r (cg_prologue)
             auto input = [&](int64_t i) -> double {
                 if (init) {
                     const double bi = S.b[i];
@@ -232,9 +262,13 @@ __global__ __launch_bounds__(256) void k_b_sweep(const Tile *__restrict__ tiles,
                     S.x[i] = 0.0;
                     return bi;
                 }
-                return cg_prologue(f, i);
+                const double ri = S.r[i] - alpha * S.q[i];
+                S.r[i] = ri;
+                S.x[i] = S.x[i] + alpha * p[i];
+                return ri;
             };
-            v = sweep_seg_run<PM, MODE, TS, TC>(input, S.z, T, D, S.n, PM, S.nparts, k, lane, lds);
+            v = sweep_seg_run<PM, MODE, TS, TC, true>(input, S.z, T, D, S.n, PM, S.nparts, k, lane,
+                                                       lds);
         }
         v = block_sum(v, sm);
         if (threadIdx.x == 0) S.part[2 * tl.tile] = v;
@@ -274,7 +308,7 @@ __global__ void k_b_setup(int64_t n, int64_t P, const int64_t *__restrict__ sptr
     const int64_t k = g / P;
     const int64_t s = k * P;
     const int64_t e = (s + P < n ? s + P : n) - 1;
-    const int64_t Pa = e - s + 1;
+    const int64_t Pa = P;  // every block full size (the last one padded, see sweep_seg_run)
     const int64_t i = g - s;
     int64_t lb, ub;
     block_bases(k, P, Pa, mode, lb, ub);
@@ -327,8 +361,7 @@ static Layout make_layout(int64_t n, const int64_t *ptr, int64_t P, int tsz, int
     }
     L.nparts = (n + P - 1) / P;
     L.ntiles = (n + TILE - 1) / TILE;
-    const int64_t Pl = n - (L.nparts - 1) * P;
-    L.tri = (L.nparts - 1) * P * (P - 1) + Pl * (Pl - 1);
+    L.tri = L.nparts * P * (P - 1);  // full blocks, the last one padded
     if (one_tri) L.tri /= 2;
     L.o_sptr = 0;
     L.o_col = align256(L.o_sptr + (L.nslices + 1) * 8);
@@ -530,15 +563,15 @@ static void host_alloc(T *&p, size_t count) {
 }
 
 template <int PM>
-static bool one_triangle_p(int prec) {
-    if (prec == HS_PREC_FP64) return use_symt<PM, double>();
-    if (prec == HS_PREC_FP32) return use_symt<PM, float>();
-    return use_symt<PM, __hip_bfloat16>();
+static bool symt_fits_p(int prec) {
+    if (prec == HS_PREC_FP64) return symt_fits<PM, double>();
+    if (prec == HS_PREC_FP32) return symt_fits<PM, float>();
+    return symt_fits<PM, __hip_bfloat16>();
 }
 
-static bool one_triangle(int P, int prec) {
-    return P == 16 ? one_triangle_p<16>(prec) : (P == 32 ? one_triangle_p<32>(prec)
-                                                         : one_triangle_p<64>(prec));
+static bool symt_fits_rt(int P, int prec) {
+    return P == 16 ? symt_fits_p<16>(prec) : (P == 32 ? symt_fits_p<32>(prec)
+                                                      : symt_fits_p<64>(prec));
 }
 
 static int resident_grid(const void *kernel, int cus) {
@@ -547,22 +580,34 @@ static int resident_grid(const void *kernel, int cus) {
     return std::max(1, nb) * cus;
 }
 
-template <int PM>
-static const void *sweep_fn_p(int prec) {
-    if (prec == HS_PREC_FP64) return reinterpret_cast<const void *>(&k_b_sweep<PM, double, double>);
-    if (prec == HS_PREC_FP32) return reinterpret_cast<const void *>(&k_b_sweep<PM, float, float>);
-    return reinterpret_cast<const void *>(&k_b_sweep<PM, __hip_bfloat16, float>);
+template <int PM, class TS, class TC>
+static const void *sweep_fn_t(bool symt) {
+    if constexpr (symt_fits<PM, TS>())
+        if (symt) return reinterpret_cast<const void *>(&k_b_sweep<PM, TS, TC, true>);
+    return reinterpret_cast<const void *>(&k_b_sweep<PM, TS, TC, false>);
 }
 
-static const void *sweep_fn(int P, int prec) {
-    return P == 16 ? sweep_fn_p<16>(prec) : (P == 32 ? sweep_fn_p<32>(prec) : sweep_fn_p<64>(prec));
+template <int PM>
+static const void *sweep_fn_p(int prec, bool symt) {
+    if (prec == HS_PREC_FP64) return sweep_fn_t<PM, double, double>(symt);
+    if (prec == HS_PREC_FP32) return sweep_fn_t<PM, float, float>(symt);
+    return sweep_fn_t<PM, __hip_bfloat16, float>(symt);
+}
+
+static const void *sweep_fn(int P, int prec, bool symt) {
+    return P == 16 ? sweep_fn_p<16>(prec, symt)
+                   : (P == 32 ? sweep_fn_p<32>(prec, symt) : sweep_fn_p<64>(prec, symt));
 }
 
 void Engine::create(const hs_batch_config &c) {
     cfg = c;
     tsz = c.precision == HS_PREC_FP64 ? 8 : (c.precision == HS_PREC_FP32 ? 4 : 2);
     dsz = c.precision == HS_PREC_FP64 ? 8 : 4;
-    one_tri = one_triangle((int)c.partsize, c.precision);
+    const bool fits = symt_fits_rt((int)c.partsize, c.precision);
+    HS_REQUIRE(c.triangles != 1 || fits, HS_EUNSUPPORTED,
+               "one-triangle storage needs partsize 16, partsize 32 in FP32/BF16 or "
+               "partsize 64 in BF16");
+    one_tri = c.triangles == 1 || (c.triangles == 0 && fits);
     int dev = 0;
     HS_CHECK_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -571,8 +616,9 @@ void Engine::create(const hs_batch_config &c) {
     // the chip at once (each walks a fixed run of the tile table; a second
     // round of workgroups would wait for the first to finish).
     const int cus = prop.multiProcessorCount;
-    grid_spmv = c.grid > 0 ? c.grid : resident_grid(reinterpret_cast<const void *>(&k_b_spmv), cus);
-    grid_sweep = c.grid > 0 ? c.grid : resident_grid(sweep_fn((int)c.partsize, c.precision), cus);
+    grid_spmv = c.grid > 0 ? c.grid : SPMV_GRID;
+    grid_sweep = c.grid > 0 ? c.grid
+                            : resident_grid(sweep_fn((int)c.partsize, c.precision, one_tri), cus);
     HS_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HS_CHECK_HIP(hipStreamCreateWithFlags(&us, hipStreamNonBlocking));
     HS_CHECK_HIP(hipEventCreateWithFlags(&ev_step, hipEventDisableTiming));
@@ -611,18 +657,29 @@ void Engine::create(const hs_batch_config &c) {
     capture();
 }
 
+template <int PM, class TS, class TC>
+static void launch_sweep_t(bool symt, unsigned grid, hipStream_t s, const Tile *tiles,
+                           const Counts *cnt, const Sys *sys, const State *st, uint8_t *arena) {
+    if constexpr (symt_fits<PM, TS>()) {
+        if (symt) {
+            hipLaunchKernelGGL((k_b_sweep<PM, TS, TC, true>), dim3(grid), dim3(TILE), 0, s, tiles,
+                               cnt, sys, st, arena);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_b_sweep<PM, TS, TC, false>), dim3(grid), dim3(TILE), 0, s, tiles, cnt,
+                       sys, st, arena);
+}
+
 template <int PM>
-static void launch_sweep(int prec, unsigned grid, hipStream_t s, const Tile *tiles,
+static void launch_sweep(int prec, bool symt, unsigned grid, hipStream_t s, const Tile *tiles,
                          const Counts *cnt, const Sys *sys, const State *st, uint8_t *arena) {
     if (prec == HS_PREC_FP64)
-        hipLaunchKernelGGL((k_b_sweep<PM, double, double>), dim3(grid), dim3(TILE), 0, s, tiles,
-                           cnt, sys, st, arena);
+        launch_sweep_t<PM, double, double>(symt, grid, s, tiles, cnt, sys, st, arena);
     else if (prec == HS_PREC_FP32)
-        hipLaunchKernelGGL((k_b_sweep<PM, float, float>), dim3(grid), dim3(TILE), 0, s, tiles, cnt,
-                           sys, st, arena);
+        launch_sweep_t<PM, float, float>(symt, grid, s, tiles, cnt, sys, st, arena);
     else
-        hipLaunchKernelGGL((k_b_sweep<PM, __hip_bfloat16, float>), dim3(grid), dim3(TILE), 0, s,
-                           tiles, cnt, sys, st, arena);
+        launch_sweep_t<PM, __hip_bfloat16, float>(symt, grid, s, tiles, cnt, sys, st, arena);
 }
 
 void Engine::enqueue_pass(hipStream_t s) {
@@ -635,11 +692,11 @@ void Engine::enqueue_pass(hipStream_t s) {
     hipLaunchKernelGGL(k_b_fin_delta, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(), cn, sy,
                        stt, arena_dev);
     if (cfg.partsize == 16)
-        launch_sweep<16>(cfg.precision, grid_sweep, s, tl, cn, sy, stt, arena_dev);
+        launch_sweep<16>(cfg.precision, one_tri, grid_sweep, s, tl, cn, sy, stt, arena_dev);
     else if (cfg.partsize == 32)
-        launch_sweep<32>(cfg.precision, grid_sweep, s, tl, cn, sy, stt, arena_dev);
+        launch_sweep<32>(cfg.precision, one_tri, grid_sweep, s, tl, cn, sy, stt, arena_dev);
     else
-        launch_sweep<64>(cfg.precision, grid_sweep, s, tl, cn, sy, stt, arena_dev);
+        launch_sweep<64>(cfg.precision, one_tri, grid_sweep, s, tl, cn, sy, stt, arena_dev);
     hipLaunchKernelGGL(k_b_fin_gamma, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(), cn, sy,
                        stt, arena_dev);
     check_launch();
@@ -876,6 +933,8 @@ int hs_batch_create(const hs_batch_config *cfg, hs_batch_t *out) {
         HS_REQUIRE(cfg->passes >= 1 && cfg->passes <= 1024, HS_EINVAL,
                    "passes must be in [1, 1024]");
         HS_REQUIRE(cfg->grid >= 0, HS_EINVAL, "grid must be >= 0");
+        HS_REQUIRE(cfg->triangles >= 0 && cfg->triangles <= 2, HS_EINVAL,
+                   "triangles must be 0 (automatic), 1 or 2");
         std::unique_ptr<hs_batch> e(new hs_batch());
         e->create(*cfg);
         *out = e.release();
@@ -972,6 +1031,17 @@ int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out) {
             HS_CHECK_HIP(hipMemcpy(out, blk + L.o_part, 2 * L.ntiles * 8, hipMemcpyDeviceToHost));
         }
     });
+}
+
+int hs_batch_busy(hs_batch_t e) {
+    if (!e || !e->step_pending) return 0;
+    const hipError_t q = hipEventQuery(e->ev_step);
+    if (q == hipErrorNotReady) return 1;
+    if (q != hipSuccess) {
+        set_error(HS_EDEVICE, std::string("hipEventQuery: ") + hipGetErrorString(q));
+        return HS_EDEVICE;
+    }
+    return 0;
 }
 
 double hs_batch_clock(hs_batch_t e) { return e ? e->now() : 0.0; }

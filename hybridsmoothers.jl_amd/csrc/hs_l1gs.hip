@@ -195,15 +195,6 @@ constexpr bool ring_ok() {
 }
 }  // namespace ring64
 
-// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
 namespace ring64 {
 struct Sched {
     int first[NCHUNK];  // first step reading chunk c

@@ -30,6 +30,15 @@ __host__ __device__ __forceinline__ int64_t colU_off(int64_t j, int64_t Pa) {
 __host__ __device__ __forceinline__ int64_t blk_stride(int64_t P, int mode) {
     return mode == MODE_SYM ? P * (P - 1) : (P * (P - 1)) / 2;
 }
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // elements of a segment's LDS copy of L in the one-triangle symmetric sweep:
 // PM rows of PM + 1 (see sweep_full_symt)
 __host__ __device__ constexpr int symt_lds(int PM) { return PM * (PM + 1); }
@@ -133,6 +142,51 @@ __device__ __forceinline__ float bcast_keep(float v, int j, float &keep) {
         keep = (lane == j) ? v : keep;
         return __shfl(v, j, PM);
     }
+}
+
+// Broadcast for segments of at most 32 lanes (several segments per wave):
+// ds_bpermute from lane j of the lane's own segment -- the step is the
+// instruction's immediate offset from base4 = 4 * (first lane of the segment)
+// -- and lane j's value kept by v_cmp_eq + v_cndmask on VCC, emitted together
+// so the compiler neither hoists 31 comparisons nor holds their masks.  Eight
+// VALU operations per recurrence step instead of thirteen with
+// readlane/writelane, which for 32-lane segments also need two moves and a
+// select (one scalar operand per VALU instruction on this chip).
+template <int J>
+__device__ __forceinline__ int keep_lane_i32(int keep, int v, int slane) {
+    asm("v_cmp_eq_u32 vcc, %3, %2\n\tv_cndmask_b32 %0, %0, %1, vcc"
+        : "+v"(keep)
+        : "v"(v), "v"(slane), "n"(J)
+        : "vcc");
+    return keep;
+}
+template <int J>
+__device__ __forceinline__ float bcast_keep_perm(float v, float &keep, int base4, int slane) {
+    keep = __int_as_float(keep_lane_i32<J>(__float_as_int(keep), __float_as_int(v), slane));
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(base4 + 4 * J, __float_as_int(v)));
+}
+template <int J>
+__device__ __forceinline__ double bcast_keep_perm(double v, double &keep, int base4, int slane) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    keep = __hiloint2double(keep_lane_i32<J>(__double2hiint(keep), hi, slane),
+                            keep_lane_i32<J>(__double2loint(keep), lo, slane));
+    return __hiloint2double(__builtin_amdgcn_ds_bpermute(base4 + 4 * J, hi),
+                            __builtin_amdgcn_ds_bpermute(base4 + 4 * J, lo));
+}
+// The full-partition recurrences' broadcast: SGPR readlane/writelane for
+// 64-lane segments, ds_bpermute for narrower ones.  `lane` is the lane's index
+// in its segment; base4 = 4 * (its segment's first lane in the wave).
+template <int PM, int J, class TC>
+__device__ __forceinline__ TC bcast_step(TC v, TC &keep, int base4, int lane) {
+    if constexpr (PM == 64) return bcast_keep<64>(v, J, keep);
+    else return bcast_keep_perm<J>(v, keep, base4, lane);
+}
+// base4 of the calling lane, opaque to the optimiser: per-step addresses
+// base4 + 4j must fold into the instruction offset, not be precomputed.
+__device__ __forceinline__ int seg_base4(int PM) {
+    int b = (int)((threadIdx.x & 63) & ~(PM - 1)) * 4;
+    asm volatile("" : "+v"(b));
+    return b;
 }
 
 // Correctly rounded z/d from the correctly rounded reciprocal r = 1/d:
@@ -246,34 +300,35 @@ __device__ __forceinline__ TC sweep_full(TC xr, TC dr, TC rr, int lane,
     TS ring[R > 0 ? R : 1];
 #pragma unroll
     for (int c = 0; c < R; ++c) ring[c] = load_col(c);
+    const int base4 = seg_base4(PM);
 
     TC yr = xr;
     if constexpr (MODE != MODE_BWD) {
         TC acc = (TC)0, yfin = (TC)0;
-#pragma unroll
-        for (int j = 0; j < PM - 1; ++j) {
+        static_for<0, PM - 1>([&](auto J) {
+            constexpr int j = decltype(J)::value;
             const TS v = ring[j % R];
-            if (j + R < NC) ring[j % R] = load_col(j + R);
+            if constexpr (j + R < NC) ring[j % R] = load_col(j + R);
             TC q = quot_fast(xr - acc, dr, rr);
-            TC yj = bcast_keep<PM>(q, j, yfin);
+            TC yj = bcast_step<PM, j>(q, yfin, base4, lane);
             acc = acc + load_as<TC>(v) * yj;  // dead for lanes <= j
-        }
+        });
         TC q = quot_fast(xr - acc, dr, rr);
         yr = (lane == PM - 1) ? q : yfin;
     }
     if constexpr (MODE == MODE_SYM) xr = dr * yr;  // y .= D_Dl1 .* y  (:897)
     if constexpr (MODE != MODE_FWD) {
         TC acc = (TC)0, yfin = (TC)0;
-#pragma unroll
-        for (int jj = 0; jj < PM - 1; ++jj) {
-            const int j = PM - 1 - jj;
-            const int c = NL + jj;
+        static_for<0, PM - 1>([&](auto JJ) {
+            constexpr int jj = decltype(JJ)::value;
+            constexpr int j = PM - 1 - jj;
+            constexpr int c = NL + jj;
             const TS v = ring[c % R];
-            if (c + R < NC) ring[c % R] = load_col(c + R);
+            if constexpr (c + R < NC) ring[c % R] = load_col(c + R);
             TC q = quot_fast(xr - acc, dr, rr);
-            TC yj = bcast_keep<PM>(q, j, yfin);
+            TC yj = bcast_step<PM, j>(q, yfin, base4, lane);
             acc = acc + load_as<TC>(v) * yj;  // dead for lanes >= j
-        }
+        });
         TC q = quot_fast(xr - acc, dr, rr);
         yr = (lane == 0) ? q : yfin;
     }
@@ -304,22 +359,23 @@ __device__ __forceinline__ TC sweep_full_symt(TC xr, TC dr, TC rr, int lane,
     };
     TS *const wrow = lds + lane * RS;  // this lane's row
     const TS *const rcol = lds + lane;  // this lane's column of every row
+    const int base4 = seg_base4(PM);
     TS ring[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) ring[c] = load_col(c);
     TC acc = (TC)0, yfin = (TC)0;
-#pragma unroll
-    for (int j = 0; j < NL; ++j) {
+    static_for<0, NL>([&](auto J) {
+        constexpr int j = decltype(J)::value;
         // keep each step's load, LDS store and arithmetic together: hoisted
         // stores would wait for their loads long before the step needs them
         __builtin_amdgcn_sched_barrier(0);
         const TS v = ring[j % R];
-        if (j + R < NL) ring[j % R] = load_col(j + R);
+        if constexpr (j + R < NL) ring[j % R] = load_col(j + R);
         wrow[j] = v;
         TC q = quot_fast(xr - acc, dr, rr);
-        TC yj = bcast_keep<PM>(q, j, yfin);
+        TC yj = bcast_step<PM, j>(q, yfin, base4, lane);
         acc = acc + load_as<TC>(v) * yj;  // dead for lanes <= j
-    }
+    });
     TC q = quot_fast(xr - acc, dr, rr);
     TC yr = (lane == PM - 1) ? q : yfin;
     const TC xb = dr * yr;  // y .= D_Dl1 .* y  (:897)
@@ -332,15 +388,15 @@ __device__ __forceinline__ TC sweep_full_symt(TC xr, TC dr, TC rr, int lane,
     for (int c = 0; c < RB; ++c) rb[c] = rcol[(PM - 1 - c) * RS];
     acc = (TC)0;
     yfin = (TC)0;
-#pragma unroll
-    for (int jj = 0; jj < NL; ++jj) {
-        const int j = PM - 1 - jj;
+    static_for<0, NL>([&](auto JJ) {
+        constexpr int jj = decltype(JJ)::value;
+        constexpr int j = PM - 1 - jj;
         const TS v = rb[jj % RB];
-        if (jj + RB < NL) rb[jj % RB] = rcol[(PM - 1 - (jj + RB)) * RS];
+        if constexpr (jj + RB < NL) rb[jj % RB] = rcol[(PM - 1 - (jj + RB)) * RS];
         TC qb = quot_fast(xb - acc, dr, rr);
-        TC yj = bcast_keep<PM>(qb, j, yfin);
+        TC yj = bcast_step<PM, j>(qb, yfin, base4, lane);
         acc = acc + load_as<TC>(v) * yj;  // dead for lanes >= j
-    }
+    });
     q = quot_fast(xb - acc, dr, rr);
     return (lane == 0) ? q : yfin;
 }
@@ -366,7 +422,12 @@ __device__ __forceinline__ double cg_prologue(const CGFuse &f, int64_t i) {
 // segment past the last partition (k >= nparts) runs the last partition's
 // recurrence on dead lanes -- its loads stay inside the buffers and it stores
 // nothing.  MODE_SYMT takes `lds`: this segment's symt_lds(PM) elements.
-template <int PM, int MODE, class TS, class TC, class In>
+// PAD (P == PM): the last partition is stored as a full block (the rows past
+// n have no couplings), and is swept as a full partition whose missing rows
+// are identity rows (d = 1, x = 0): they stay exactly 0 and add exact zero
+// terms to the real rows, whose results are therefore those of the short
+// partition's own recurrence.
+template <int PM, int MODE, class TS, class TC, bool PAD, class In>
 __device__ __forceinline__ double sweep_seg_run(In &&input, double *y, const TS *__restrict__ T,
                                                 const TC *__restrict__ D, int64_t n, int P,
                                                 int64_t nparts, int64_t k, int lane,
@@ -375,10 +436,11 @@ __device__ __forceinline__ double sweep_seg_run(In &&input, double *y, const TS 
     const int64_t kk = valid ? k : nparts - 1;
     const int64_t s = kk * (int64_t)P;
     const int64_t rem = n - s;
-    const int Pa = (int)(rem < P ? rem : P);
+    const int Pr = (int)(rem < P ? rem : P);  // rows of this partition
+    const int Pa = PAD ? PM : Pr;             // rows of its stored block
     int64_t lb, ub;
     block_bases(kk, P, Pa, MODE, lb, ub);
-    const bool live = valid && lane < Pa;
+    const bool live = valid && lane < Pr;
     const TS *Lk = T + lb;
     const TS *Uk = T + ub;
 
@@ -392,7 +454,7 @@ __device__ __forceinline__ double sweep_seg_run(In &&input, double *y, const TS 
         yr = sweep_generic<PM, MODE, false, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
     } else if (__all(Pa == PM)) {
         if constexpr (MODE == MODE_SYMT)
-            yr = sweep_full_symt<PM, 16, TS, TC>(xr, dr, rr, lane, Lk, lds);
+            yr = sweep_full_symt<PM, 32, TS, TC>(xr, dr, rr, lane, Lk, lds);
         else
             yr = sweep_full<PM, MODE, 16, TS, TC>(xr, dr, rr, lane, Lk, Uk);
     } else {
@@ -411,11 +473,11 @@ __device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
                                                  int64_t nparts, int64_t k, int lane,
                                                  const CGFuse &f, TS *lds = nullptr) {
     if constexpr (CG)
-        return sweep_seg_run<PM, MODE, TS, TC>([&](int64_t i) { return cg_prologue(f, i); }, y, T,
-                                               D, n, P, nparts, k, lane, lds);
+        return sweep_seg_run<PM, MODE, TS, TC, false>([&](int64_t i) { return cg_prologue(f, i); },
+                                                      y, T, D, n, P, nparts, k, lane, lds);
     else
-        return sweep_seg_run<PM, MODE, TS, TC>([&](int64_t i) { return x[i]; }, y, T, D, n, P,
-                                               nparts, k, lane, lds);
+        return sweep_seg_run<PM, MODE, TS, TC, false>([&](int64_t i) { return x[i]; }, y, T, D, n,
+                                                      P, nparts, k, lane, lds);
 }
 
 }  // namespace hs

@@ -64,7 +64,7 @@ class BatchConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_int64), ("max_systems", C.c_int64),
                 ("staging_bytes", C.c_int64), ("partsize", C.c_int64),
                 ("precision", C.c_int), ("passes", C.c_int), ("grid", C.c_int),
-                ("eta", C.c_double)]
+                ("eta", C.c_double), ("triangles", C.c_int)]
 
 
 class BatchResult(C.Structure):
@@ -103,7 +103,7 @@ EXPORTED = [
     "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve",
     "hs_problem_lognormal",
     "hs_batch_create", "hs_batch_footprint", "hs_batch_submit", "hs_batch_step",
-    "hs_batch_collect", "hs_batch_info", "hs_batch_clock", "hs_batch_peek", "hs_batch_destroy",
+    "hs_batch_collect", "hs_batch_info", "hs_batch_clock", "hs_batch_peek", "hs_batch_busy", "hs_batch_destroy",
 ]
 
 
@@ -138,6 +138,7 @@ def bind_optional():
         _sig("hs_batch_info", [vp, vp])
         _sig("hs_batch_clock", [vp], C.c_double)
         _sig("hs_batch_peek", [vp, i64, C.c_int, vp])
+        _sig("hs_batch_busy", [vp])
         _sig("hs_batch_destroy", [vp])
 
 

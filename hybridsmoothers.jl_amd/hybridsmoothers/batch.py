@@ -87,8 +87,7 @@ def iteration_bytes(ptr, partsize: int, precision: str, triangles: int = 2) -> i
     dsz = 8 if precision == "fp64" else 4
     P = partsize
     nparts = (n + P - 1) // P
-    Pl = n - (nparts - 1) * P
-    tri = ((nparts - 1) * P * (P - 1) + Pl * (Pl - 1)) * triangles // 2
+    tri = nparts * P * (P - 1) * triangles // 2  # the last block is stored full size
     return nent * 12 + (w.size + 1) * 8 + tri * tsz + n * (dsz + 88)
 
 
@@ -104,14 +103,15 @@ def default_arena_bytes(fraction: float = 0.5, cap: int = 96 << 30) -> int:
 class BatchSolver:
     def __init__(self, partsize: int = 32, precision: str = "fp32", arena_bytes: int | None = None,
                  max_systems: int = 1024, staging_bytes: int = 256 << 20, passes: int = 8,
-                 grid: int = 0, eta: float = 1.5, device: int = 0):
+                 grid: int = 0, eta: float = 1.5, triangles: int = 0, device: int = 0):
         L.init(device)
         if precision not in _PREC:
             raise L.HSError(L.HS_EINVAL, f"unknown precision {precision!r}")
         cfg = L.BatchConfig(arena_bytes=int(arena_bytes or default_arena_bytes()),
                             max_systems=int(max_systems), staging_bytes=int(staging_bytes),
                             partsize=int(partsize), precision=_PREC[precision],
-                            passes=int(passes), grid=int(grid), eta=float(eta))
+                            passes=int(passes), grid=int(grid), eta=float(eta),
+                            triangles=int(triangles))
         self.config = cfg
         self.partsize, self.precision, self.passes = int(partsize), precision, int(passes)
         self._h = C.c_void_p()
@@ -163,6 +163,13 @@ class BatchSolver:
                                          r.t_done, tag))
             if cnt.value < len(self._res):
                 return out
+
+    def busy(self) -> bool:
+        """True while the last step is still running on the device."""
+        r = L.lib.hs_batch_busy(self._h)
+        if r < 0:
+            L.check(r)
+        return r == 1
 
     def pending(self) -> int:
         """Systems submitted and not yet collected."""

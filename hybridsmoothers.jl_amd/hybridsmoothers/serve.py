@@ -9,11 +9,13 @@ loop repeatedly
   1. moves requests whose arrival time has passed into the pending queue;
   2. launches one engine step (a graph of `passes` CG iterations of every
      resident system) without waiting for it;
-  3. while the device works, admits pending requests in arrival order (with a
-     small look-ahead window so one large system does not block smaller ones
-     behind it) until the engine refuses one for lack of slots or memory --
-     each admission uploads the system and builds its preconditioner on a
-     second stream, overlapping the step;
+  3. while the device works on that step (and at least once per round),
+     admits pending requests in arrival order (with a small look-ahead window
+     so one large system does not block smaller ones behind it) until the
+     engine refuses one for lack of slots or memory -- each admission uploads
+     the system and builds its preconditioner on a second stream; admissions
+     stop when the step finishes, so the device never waits for a long
+     admission burst;
   4. collects the systems that finished (their memory is freed at once, which
      is what lets the next requests in).
 
@@ -142,7 +144,8 @@ def generate(specs, rhs_seed: int = 1000, workers: int = 8):
 
 
 def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
-          producer=None, window: int = 16, keep_x: bool = False, on_result=None):
+          producer=None, window: int = 16, keep_x: bool = False, on_result=None,
+          admit_idle_s: float = 0.005):
     """Runs the stream through `engine`; returns (ServeReport, per-request records).
 
     `producer` supplies request i's (matrix, b) through get(i): a Producer
@@ -173,9 +176,17 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
             busy = engine.pending() > 0
             if busy:
                 engine.step()
-            # admission while the step runs
+            # admission while the step runs; with nothing on the device, for a
+            # few milliseconds before the first step starts (at least one
+            # attempt per round either way)
             k = 0
+            tried = 0
+            t_adm = time.perf_counter()
             while k < min(window, len(pending)):
+                if tried and (not engine.busy() if busy
+                              else time.perf_counter() - t_adm > admit_idle_s):
+                    break
+                tried += 1
                 i = pending[k]
                 if i not in staged:
                     staged[i] = prod.get(i)

@@ -191,6 +191,9 @@ typedef struct {
     int passes;             /* CG iterations per step (one graph launch) */
     int grid;               /* workgroups of the tile kernels; 0 = as many as are resident at once */
     double eta;             /* the l1 rule's eta (reference default 1.5) */
+    int triangles;          /* packed triangles stored per partition: 1 the lower only (U = L^T,
+                               half the bytes; partsize 16, 32 in FP32/BF16, 64 in BF16),
+                               2 both, 0 automatic (1 where possible) */
 } hs_batch_config;
 
 typedef struct {
@@ -233,6 +236,10 @@ int hs_batch_info(hs_batch_t e, int64_t *out);
  * 5 D_Dl1 (n, widened to FP64), 6 the two partials of each 256-row tile.
  * For tests and diagnostics. */
 int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out);
+/* 1 while the last step is still running on the device, 0 once it has
+ * finished (or none is pending), negative on a device error.  Lets a caller
+ * admit new systems exactly while the device is busy. */
+int hs_batch_busy(hs_batch_t e);
 /* Seconds since the engine was created (the clock of hs_batch_result). */
 double hs_batch_clock(hs_batch_t e);
 int hs_batch_destroy(hs_batch_t e);
