@@ -121,7 +121,7 @@ def serving_bench(args, log_fn=log):
     specs = pb.serving_stream(N, seed=5)
     log_fn(f"[bench] serving: autotune on a 48-system sample")
     sample = autotune.sample_systems(pb.serving_stream(48, seed=77))
-    tune = autotune.tune_batch(sample, partsizes=(32,), passes=(4, 8, 16), grids=(0, 1024),
+    tune = autotune.tune_batch(sample, partsizes=(32,), passes=(8, 16, 32), grids=(0, 2048),
                                precision="fp32", arena_bytes=32 << 30, log=log_fn)
     out = {"workload": f"{N} SPD systems, N log-uniform in [1e3, 2.6e5] (2D 5-point / 3D "
                        f"7-point lognormal sigma=1, alternating), b = U[-1,1]; FP64 CG, FP32 "
@@ -149,6 +149,61 @@ def serving_bench(args, log_fn=log):
     out["latency_p99_ms"] = round(rep.latency_p99_ms, 3)
     out["capacity_solves_per_s"] = round(rep_sat.solves_per_s, 2)
     return out
+
+
+def mixed_precision_bench(hs, pb, builder, n3=192, log_fn=log):
+    """3D 7-point variable-coefficient Poisson on n3^3 with lognormal face
+    coefficients (seed 2), b = U[-1,1] (seed 2): FP64 CG to rtol 1e-8 with an
+    L1GS partsize-64 symmetric smoother in FP64, FP32 and BF16 (off-diagonals),
+    each reported with its final true residual and its iteration delta
+    against the all-FP64 solve."""
+    import torch
+    log_fn(f"[bench] mixed precision: building lognormal {n3}^3")
+    A = pb.lognormal3d(n3, seed=2)
+    n = A.shape[0]
+    M = hs.DeviceMatrix(A)
+    b = torch.tensor(pb.uniform_rhs(n, 2), device="cuda")
+    rows = []
+    for prec in ("fp64", "fp32", "bf16"):
+        log_fn(f"[bench] mixed precision: {prec} smoother")
+        P = builder(A, 64, isSymA=True, sweep=hs.SymmetricSweep(), precision=prec)
+        rows.append(pcg_bench(hs, M, b, P, f"L1GS P=64 symmetric, {prec.upper()} smoother",
+                              reps=1))
+        del P
+    base = rows[0]
+    for r in rows:
+        r["iteration_delta"] = r["iters"] - base["iters"]
+        r["speedup_vs_fp64"] = round(r["solves_per_s"] / base["solves_per_s"], 3)
+    return {"workload": f"3D 7-point lognormal-coefficient Poisson {n3}^3 (N={n}, "
+                        f"nnz={A.nnz}), seed 2; FP64 CG rtol 1e-8", "solves": rows}
+
+
+def batch_steady(kind, nsys=256, iters=60):
+    """Steady state of the batched engine for a kernel profile: `nsys` equal
+    systems (2D 181^2 or 3D 32^3 lognormal, 8.4 M rows in all) resident
+    together, FP32 L1GS partsize 32, a fixed `iters` CG iterations."""
+    from hybridsmoothers import problems as pb
+    from hybridsmoothers.batch import BatchSolver, iteration_bytes
+    dims = (181, 181, 1) if kind == "2d" else (32, 32, 32)
+    systems = []
+    for i in range(nsys):
+        A = pb.lognormal_grid(*dims, seed=100 + i)
+        systems.append((A, pb.uniform_rhs(A[0].size - 1, 7 + i)))
+    with BatchSolver(partsize=32, precision="fp32", passes=8, max_systems=1024) as eng:
+        for A, b in systems:
+            if eng.submit(A, b, itmax=iters) is None:
+                raise RuntimeError("engine full")
+        t0 = time.perf_counter()
+        sols = []
+        while eng.pending():
+            eng.step()
+            sols += eng.collect()
+        dt = time.perf_counter() - t0
+        ib = iteration_bytes(systems[0][0][0], 32, "fp32", eng.triangles)
+    rows = nsys * systems[0][1].size
+    return {"workload": f"{nsys} x lognormal {'x'.join(map(str, dims))}", "rows": rows,
+            "iterations": sols[0].iters, "bytes_per_iteration": ib * nsys, "wall_s": round(dt, 4),
+            "GB/s_whole_run": round(ib * nsys * (sols[0].iters + 1) / dt / 1e9, 1)}
 
 
 def serving_cpu_baseline(budget_s=10.0):
@@ -189,6 +244,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=1024)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true", help="only the serving benchmark")
+    ap.add_argument("--batch-steady", choices=("2d", "3d"), default=None,
+                    help="only the batched engine's steady state (for kernel profiles)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 192^3 mixed-precision PCG")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB per step (counter calibration)")
     args = ap.parse_args()
@@ -212,6 +270,9 @@ def main():
     from hybridsmoothers import problems as pb
 
     hs.init(0)
+    if args.batch_steady:
+        print(json.dumps({"batch_steady": batch_steady(args.batch_steady)}), flush=True)
+        return
     if args.serve_only:
         res = serving_bench(args)
         if not args.no_cpu:
@@ -315,6 +376,9 @@ def main():
         for r in pcg:
             r["speedup_vs_fp64_l1gs"] = round(r["solves_per_s"] / base, 3)
         out["pcg_128"] = pcg
+        del M
+        if not args.no_mixed:
+            out["mixed_precision_192"] = mixed_precision_bench(hs, pb, builder)
     if not args.quick and not args.no_serve:
         out["serving"] = serving_bench(args)
         if not args.no_cpu:

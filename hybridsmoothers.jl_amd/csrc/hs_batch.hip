@@ -612,11 +612,11 @@ void Engine::create(const hs_batch_config &c) {
     HS_CHECK_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HS_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
-    // Persistent tile kernels: by default exactly as many workgroups as fit on
+    // The persistent sweep: by default exactly as many workgroups as fit on
     // the chip at once (each walks a fixed run of the tile table; a second
     // round of workgroups would wait for the first to finish).
     const int cus = prop.multiProcessorCount;
-    grid_spmv = c.grid > 0 ? c.grid : SPMV_GRID;
+    grid_spmv = SPMV_GRID;
     grid_sweep = c.grid > 0 ? c.grid
                             : resident_grid(sweep_fn((int)c.partsize, c.precision, one_tri), cus);
     HS_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));

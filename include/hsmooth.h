@@ -189,7 +189,8 @@ typedef struct {
     int64_t partsize;       /* 16, 32 or 64 */
     int precision;          /* smoother storage: HS_PREC_FP64, HS_PREC_FP32 or HS_PREC_BF16 */
     int passes;             /* CG iterations per step (one graph launch) */
-    int grid;               /* workgroups of the tile kernels; 0 = as many as are resident at once */
+    int grid;               /* workgroups of the persistent sweep kernel; 0 = as many as are
+                               resident at once (the SpMV always uses one per 256-row tile) */
     double eta;             /* the l1 rule's eta (reference default 1.5) */
     int triangles;          /* packed triangles stored per partition: 1 the lower only (U = L^T,
                                half the bytes; partsize 16, 32 in FP32/BF16, 64 in BF16),
