@@ -32,6 +32,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <cstddef>
 #include <memory>
 #include <type_traits>
@@ -938,6 +940,155 @@ int hs_l1gs_set_variant(hs_l1gs_t h, int variant) {
         HS_REQUIRE(h, HS_EINVAL, "NULL preconditioner handle");
         HS_REQUIRE(variant >= 0 && variant <= 2, HS_EINVAL, "unknown kernel variant");
         h->variant = variant;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// persisted state: the built preconditioner as a file, reloaded without the
+// matrix.  Layout (little-endian): a fixed header, then the arrays in header
+// order, each exactly as stored on the device; the header ends with an
+// FNV-1a 64 checksum of everything after it.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr char kMagic[8] = {'H', 'S', 'L', '1', 'G', 'S', '0', '1'};
+
+struct FileHeader {
+    char magic[8];
+    int64_t n, P, nparts, nchunks, chunksize, S;
+    int32_t sweep, cache, prec, fmt, sym, rowwise, variant, pad;
+    int64_t nbytes[7];  // D, Df, T, vptr, vind, vdiag, vval
+    uint64_t checksum;  // FNV-1a 64 of the payload
+};
+
+uint64_t fnv1a(uint64_t h, const uint8_t *p, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 0x100000001b3ULL;
+    }
+    return h;
+}
+
+struct FileCloser {
+    void operator()(std::FILE *f) const {
+        if (f) std::fclose(f);
+    }
+};
+
+}  // namespace
+
+int hs_l1gs_save(hs_l1gs_t h, const char *path) {
+    return guard([&] {
+        HS_REQUIRE(h && path, HS_EINVAL, "NULL argument");
+        FileHeader hd{};
+        std::memcpy(hd.magic, kMagic, 8);
+        hd.n = h->n;
+        hd.P = h->P;
+        hd.nparts = h->nparts;
+        hd.nchunks = h->nchunks;
+        hd.chunksize = h->chunksize;
+        hd.S = h->S;
+        hd.sweep = h->sweep;
+        hd.cache = h->cache;
+        hd.prec = h->prec;
+        hd.fmt = h->fmt;
+        hd.sym = h->sym;
+        hd.rowwise = h->rowwise;
+        hd.variant = h->variant;
+        const void *dev[7] = {h->D.get(), h->Df.get(), h->T.get(), h->vptr.get(),
+                              h->vind.get(), h->vdiag.get(), h->vval.get()};
+        const size_t sizes[7] = {h->D.bytes(), h->Df.bytes(), h->T.bytes(), h->vptr.bytes(),
+                                 h->vind.bytes(), h->vdiag.bytes(), h->vval.bytes()};
+        std::vector<std::vector<uint8_t>> host(7);
+        uint64_t ck = 0xcbf29ce484222325ULL;
+        for (int k = 0; k < 7; ++k) {
+            hd.nbytes[k] = (int64_t)sizes[k];
+            host[k].resize(sizes[k]);
+            if (sizes[k])
+                HS_CHECK_HIP(hipMemcpy(host[k].data(), dev[k], sizes[k], hipMemcpyDeviceToHost));
+            ck = fnv1a(ck, host[k].data(), sizes[k]);
+        }
+        hd.checksum = ck;
+        std::unique_ptr<std::FILE, FileCloser> f(std::fopen(path, "wb"));
+        HS_REQUIRE(f, HS_EINVAL, std::string("cannot open for writing: ") + path);
+        bool ok = std::fwrite(&hd, sizeof hd, 1, f.get()) == 1;
+        for (int k = 0; k < 7 && ok; ++k)
+            ok = sizes[k] == 0 || std::fwrite(host[k].data(), 1, sizes[k], f.get()) == sizes[k];
+        ok = ok && std::fflush(f.get()) == 0;
+        HS_REQUIRE(ok, HS_EINVAL, std::string("write failed: ") + path);
+    });
+}
+
+int hs_l1gs_load(const char *path, void *stream, hs_l1gs_t *out) {
+    return guard([&] {
+        HS_REQUIRE(path && out, HS_EINVAL, "NULL argument");
+        *out = nullptr;
+        std::unique_ptr<std::FILE, FileCloser> f(std::fopen(path, "rb"));
+        HS_REQUIRE(f, HS_EINVAL, std::string("cannot open: ") + path);
+        FileHeader hd{};
+        HS_REQUIRE(std::fread(&hd, sizeof hd, 1, f.get()) == 1, HS_EINVAL,
+                   "not a saved L1GS preconditioner (short header)");
+        HS_REQUIRE(std::memcmp(hd.magic, kMagic, 8) == 0, HS_EINVAL,
+                   "not a saved L1GS preconditioner (bad magic)");
+        const int64_t n = hd.n;
+        HS_REQUIRE(n >= 0 && hd.P > 0 && hd.sweep >= 0 && hd.sweep <= 2 &&
+                       (hd.cache == HS_CACHE_PACKED || hd.cache == HS_CACHE_VIEW) &&
+                       hd.prec >= HS_PREC_FP64 && hd.prec <= HS_PREC_BF16 && hd.variant >= 0 &&
+                       hd.variant <= 2,
+                   HS_EINVAL, "corrupt L1GS file (header fields)");
+        const int64_t nparts = n > 0 ? (n + hd.P - 1) / hd.P : 0;
+        const int64_t elem = hd.prec == HS_PREC_FP64 ? 8 : (hd.prec == HS_PREC_FP32 ? 4 : 2);
+        const int64_t tri_elems = hd.cache == HS_CACHE_PACKED
+                                      ? (hd.sweep == HS_SWEEP_SYMMETRIC ? 2 * hd.S : hd.S)
+                                      : 0;
+        HS_REQUIRE(hd.nparts == nparts && hd.nbytes[0] == n * 8 &&
+                       hd.nbytes[1] == (hd.prec == HS_PREC_FP64 ? 0 : n * 4) &&
+                       hd.nbytes[2] == tri_elems * elem,
+                   HS_EINVAL, "corrupt L1GS file (array sizes)");
+        for (int k = 3; k < 7; ++k)
+            HS_REQUIRE(hd.nbytes[k] >= 0 && hd.nbytes[k] % 8 == 0, HS_EINVAL,
+                       "corrupt L1GS file (array sizes)");
+        std::vector<std::vector<uint8_t>> host(7);
+        uint64_t ck = 0xcbf29ce484222325ULL;
+        for (int k = 0; k < 7; ++k) {
+            host[k].resize((size_t)hd.nbytes[k]);
+            HS_REQUIRE(hd.nbytes[k] == 0 ||
+                           std::fread(host[k].data(), 1, host[k].size(), f.get()) == host[k].size(),
+                       HS_EINVAL, "truncated L1GS file");
+            ck = fnv1a(ck, host[k].data(), host[k].size());
+        }
+        HS_REQUIRE(ck == hd.checksum, HS_EINVAL, "corrupt L1GS file (checksum mismatch)");
+        hipStream_t st = as_stream(stream);
+        std::unique_ptr<hs_l1gs> h(new hs_l1gs());
+        h->n = n;
+        h->P = hd.P;
+        h->nparts = hd.nparts;
+        h->nchunks = hd.nchunks;
+        h->chunksize = hd.chunksize;
+        h->S = hd.S;
+        h->sweep = hd.sweep;
+        h->cache = hd.cache;
+        h->prec = hd.prec;
+        h->fmt = hd.fmt;
+        h->sym = hd.sym;
+        h->rowwise = hd.rowwise;
+        h->variant = hd.variant;
+        auto up = [&](auto &buf, int k) {
+            using T = std::remove_reference_t<decltype(*buf.get())>;
+            buf.alloc((size_t)hd.nbytes[k] / sizeof(T));
+            if (hd.nbytes[k])
+                HS_CHECK_HIP(hipMemcpyAsync(buf.get(), host[k].data(), (size_t)hd.nbytes[k],
+                                            hipMemcpyHostToDevice, st));
+        };
+        up(h->D, 0);
+        up(h->Df, 1);
+        up(h->T, 2);
+        up(h->vptr, 3);
+        up(h->vind, 4);
+        up(h->vdiag, 5);
+        up(h->vval, 6);
+        HS_CHECK_HIP(hipStreamSynchronize(st));  // the host buffers go out of scope
+        *out = h.release();
     });
 }
 

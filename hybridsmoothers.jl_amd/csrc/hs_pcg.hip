@@ -379,6 +379,37 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
 
 void cgwork_free(CGWork *w) { delete w; }
 
+// ---------------------------------------------------------------------------
+// the AMG smoother form: `iters` times x -= M^-1 (A x - b)
+// (ext/HybridSmoothersAMGExt.jl:29-35: mul!(residual, A, x); residual .-= b;
+// ldiv!(correction, P, residual); x .-= correction)
+// ---------------------------------------------------------------------------
+// r_i = (A x)_i - b_i: the row sum in column order first (mul! with a zero
+// start), then the subtraction, as the reference rounds them.
+__global__ void k_resid_minus(SellView<double> A, const double *__restrict__ x,
+                              const double *__restrict__ b, double *__restrict__ r) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A.n) r[i] = row_dot(A, x, i) - b[i];
+}
+
+__global__ void k_sub_inplace(int64_t n, double *__restrict__ x, const double *__restrict__ z) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] - z[i];
+}
+
+void l1gs_smooth_device(const CSR &A, const L1GSPlan &P, double *x, const double *b, int iters,
+                        double *r, double *z, hipStream_t st) {
+    const int64_t n = A.n;
+    const unsigned g = grid_for(n, 256);
+    for (int it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(k_resid_minus, dim3(g), dim3(256), 0, st, A.view(), x, b, r);
+        check_launch();
+        l1gs_apply_device(P, r, z, st, nullptr);
+        hipLaunchKernelGGL(k_sub_inplace, dim3(g), dim3(256), 0, st, n, x, z);
+        check_launch();
+    }
+}
+
 // The CG workspace of a matrix handle: created on first use, freed with the
 // handle.  Concurrent solves with one matrix handle are not supported.
 static CGWork &work_for(CSR &A) {
@@ -391,6 +422,36 @@ static CGWork &work_for(CSR &A) {
 using namespace hs;
 
 extern "C" {
+
+int hs_l1gs_smooth(hs_l1gs_t P, hs_csr_t A, double *x, const double *b, int iters, int loc,
+                   void *stream, int flags) {
+    return guard([&] {
+        HS_REQUIRE(P && A, HS_EINVAL, "NULL handle");
+        HS_REQUIRE(iters >= 0, HS_EINVAL, "iterations must be non-negative");
+        const int64_t n = A->n;
+        // hs_l1gs is the plan type of hs_l1gs.hip (a complete type only there)
+        const L1GSPlan &plan = *reinterpret_cast<const L1GSPlan *>(P);
+        HS_REQUIRE(l1gs_rows(plan) == n, HS_EINVAL, "preconditioner size does not match the matrix");
+        HS_REQUIRE(n == 0 || (x && b), HS_EINVAL, "NULL vector");
+        if (n == 0 || iters == 0) return;
+        hipStream_t st = as_stream(stream);
+        // the residual and correction vectors of the smoother (HybridSmoother's
+        // residual/correction fields), kept with the matrix between calls
+        CGWork &w = work_for(*A);
+        w.ensure(n);
+        if (loc == HS_LOC_DEVICE) {
+            l1gs_smooth_device(*A, plan, x, b, iters, w.r.get(), w.z.get(), st);
+            finish(st, flags);
+            return;
+        }
+        DBuf<double> dx(n);
+        HS_CHECK_HIP(hipMemcpyAsync(dx.get(), x, n * 8, hipMemcpyHostToDevice, st));
+        HS_CHECK_HIP(hipMemcpyAsync(w.b.get(), b, n * 8, hipMemcpyHostToDevice, st));
+        l1gs_smooth_device(*A, plan, dx.get(), w.b.get(), iters, w.r.get(), w.z.get(), st);
+        HS_CHECK_HIP(hipMemcpyAsync(x, dx.get(), n * 8, hipMemcpyDeviceToHost, st));
+        HS_CHECK_HIP(hipStreamSynchronize(st));
+    });
+}
 
 int hs_pcg_solve(hs_csr_t A, int prec_kind, void *prec, const double *b, double *x, double rtol,
                  double atol, int64_t itmax, int loc, void *stream, hs_pcg_stats *stats) {

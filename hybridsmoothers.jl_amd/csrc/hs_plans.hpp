@@ -149,4 +149,8 @@ uint64_t cheb_serial(const ChebPlan &s);
 
 void spmv_device(const CSR &A, const double *x, double *y, hipStream_t st);
 
+// `iters` times x -= M^-1 (A x - b) on device vectors (r, z: n-vectors of scratch).
+void l1gs_smooth_device(const CSR &A, const L1GSPlan &P, double *x, const double *b, int iters,
+                        double *r, double *z, hipStream_t st);
+
 }  // namespace hs

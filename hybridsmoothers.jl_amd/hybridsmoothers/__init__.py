@@ -15,6 +15,7 @@ from .l1gs import (  # noqa: F401
     Symmetric, SymmetricSweep, backslash, functional,
 )
 from .l1gs import ldiv_ as _l1gs_ldiv_
+from .l1gs import load as load_preconditioner  # noqa: F401
 from .pcg import DeviceMatrix, KrylovJL_CG, LinearProblem, Solution, pcg_solve, solve  # noqa: F401
 from .batch import BatchSolution, BatchSolver  # noqa: F401
 

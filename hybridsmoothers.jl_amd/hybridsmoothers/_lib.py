@@ -93,14 +93,16 @@ _sig("hs_l1gs_info", [vp, vp])
 _sig("hs_l1gs_get", [vp, C.c_int, vp])
 _sig("hs_l1gs_destroy", [vp])
 _sig("hs_l1gs_set_variant", [vp, C.c_int])
+_sig("hs_l1gs_save", [vp, C.c_char_p])
+_sig("hs_l1gs_load", [C.c_char_p, vp, C.POINTER(vp)])
 
 EXPORTED = [
     "hs_init", "hs_last_error", "hs_last_error_aux", "hs_device_bytes",
     "hs_l1gs_build", "hs_l1gs_apply", "hs_l1gs_info", "hs_l1gs_get", "hs_l1gs_destroy",
-    "hs_l1gs_set_variant",
+    "hs_l1gs_set_variant", "hs_l1gs_save", "hs_l1gs_load",
     "hs_csr_create", "hs_csr_spmv", "hs_csr_info", "hs_csr_get", "hs_csr_destroy",
     "hs_cheb_build", "hs_cheb_smooth", "hs_cheb_apply", "hs_cheb_info", "hs_cheb_get",
-    "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve",
+    "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve", "hs_l1gs_smooth",
     "hs_problem_lognormal",
     "hs_batch_create", "hs_batch_footprint", "hs_batch_submit", "hs_batch_step",
     "hs_batch_collect", "hs_batch_info", "hs_batch_clock", "hs_batch_peek", "hs_batch_busy", "hs_batch_destroy",
@@ -127,6 +129,8 @@ def bind_optional():
     if hasattr(lib, "hs_pcg_solve"):
         _sig("hs_pcg_solve", [vp, C.c_int, vp, vp, vp, C.c_double, C.c_double, i64, C.c_int, vp,
                               C.POINTER(PCGStats)])
+    if hasattr(lib, "hs_l1gs_smooth"):
+        _sig("hs_l1gs_smooth", [vp, vp, vp, vp, C.c_int, C.c_int, vp, C.c_int])
     if hasattr(lib, "hs_problem_lognormal"):
         _sig("hs_problem_lognormal", [i64, i64, i64, C.c_double, C.c_uint64, vp, vp, vp, vp])
     if hasattr(lib, "hs_batch_create"):

@@ -15,6 +15,8 @@ raises instead of silently running anything on the host.
 from __future__ import annotations
 
 import ctypes as C
+import os
+import struct
 from dataclasses import dataclass
 
 import numpy as np
@@ -327,6 +329,10 @@ class L1GSPreconditioner:
         """Selects the kernel variant of the partsize-64 symmetric sweep (0-4)."""
         L.check(L.lib.hs_l1gs_set_variant(self._h, int(variant)))
 
+    def save(self, path):
+        """Writes the built preconditioner to `path` (see load)."""
+        L.check(L.lib.hs_l1gs_save(self._h, os.fsencode(path)))
+
     def _get(self, what):
         size = self.n if what == L.GET_D_DL1 else self.packed_size
         out = np.zeros(size)
@@ -342,6 +348,29 @@ class L1GSPreconditioner:
     def __repr__(self):
         return (f"L1GSPreconditioner({type(self.sweep_kind).__name__[:-5]}, ROCBackend, "
                 f"nparts={self.partitioning.nparts})")
+
+
+_SWEEPS = {L.SWEEP_FORWARD: ForwardSweep, L.SWEEP_BACKWARD: BackwardSweep,
+           L.SWEEP_SYMMETRIC: SymmetricSweep}
+_CACHES = {L.CACHE_PACKED: PackedBufferCache, L.CACHE_VIEW: MatrixViewCache}
+
+
+def load(path, backend=None, stream=None) -> L1GSPreconditioner:
+    """Recreates a preconditioner written by L1GSPreconditioner.save on the
+    device, without the matrix and without rebuilding it."""
+    backend = backend or ROCBackend()
+    if not functional(backend):
+        raise L.HSError(L.HS_ENODEVICE, f" {backend} is not functional, please check your backend.")
+    h = C.c_void_p()
+    st = stream if stream is not None else L.current_stream()
+    L.check(L.lib.hs_l1gs_load(os.fsencode(path), st, C.byref(h)))
+    info = np.zeros(8, np.int64)
+    L.check(L.lib.hs_l1gs_info(h, info.ctypes.data))
+    with open(path, "rb") as f:  # header: magic, 6 x int64, then sweep, cache, precision
+        fields = struct.unpack("<8s6q3i", f.read(8 + 48 + 12))
+    precision = {v: k for k, v in _PREC.items()}[fields[9]]
+    return L1GSPreconditioner(h, backend, _SWEEPS[int(info[6])](), _CACHES[int(info[7])](),
+                              precision)
 
 
 def ldiv_(y, P: L1GSPreconditioner, x, stream=None, sync=True):

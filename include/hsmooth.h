@@ -100,6 +100,14 @@ int hs_l1gs_get(hs_l1gs_t h, int what, double *out);
  * LDS ring of 8 / 16 chunks, 12 / 24 steps ahead (2 is the default).  Used by
  * the autotuner.  No counterpart in the reference (one kernel per sweep). */
 int hs_l1gs_set_variant(hs_l1gs_t h, int variant);
+/* Persisted state (no counterpart in the reference, which rebuilds; the
+ * serving use in SURVEY's next steps): writes the built preconditioner --
+ * partitioning, D_Dl1, packed triangles and (MatrixViewCache) the device
+ * matrix, exactly as stored on the device -- to `path` with a checksum, and
+ * recreates it on the device without the matrix or a rebuild.  Loading a
+ * truncated, corrupted or foreign file fails with HS_EINVAL. */
+int hs_l1gs_save(hs_l1gs_t h, const char *path);
+int hs_l1gs_load(const char *path, void *stream, hs_l1gs_t *out);
 int hs_l1gs_destroy(hs_l1gs_t h);
 
 /* ---- device sparse operator (CSR) ------------------------------------- */
@@ -131,6 +139,16 @@ int hs_cheb_info(hs_cheb_t s, double *out);
 int hs_cheb_get(hs_cheb_t s, int what, double *out);
 hs_csr_t hs_cheb_matrix(hs_cheb_t s);
 int hs_cheb_destroy(hs_cheb_t s);
+
+/* ---- AMG smoother glue -------------------------------------------------- */
+/*
+ * AMG.smooth!(x, s::HybridSmoother, b) (ext/HybridSmoothersAMGExt.jl:29-35):
+ * `iters` times { r = A x - b; z = M^-1 r; x -= z } with the L1GS
+ * preconditioner P (any sweep and layout) and the device matrix A; the
+ * reference runs 2 iterations.  x is updated in place; loc as elsewhere.
+ */
+int hs_l1gs_smooth(hs_l1gs_t P, hs_csr_t A, double *x, const double *b, int iters, int loc,
+                   void *stream, int flags);
 
 /* ---- preconditioned CG ------------------------------------------------ */
 #define HS_PCG_NONE 0

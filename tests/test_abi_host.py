@@ -173,3 +173,28 @@ def test_problem_generators():
     specs = pb.serving_stream(64, seed=5)
     sizes = [g * g if k == "2d" else g ** 3 for k, g, _ in specs]
     assert min(sizes) >= 700 and max(sizes) <= 3.0e5
+
+
+def test_l1gs_load_rejects_bad_files(tmp_path):
+    """hs_l1gs_load checks the file before any device work: missing file,
+    foreign magic, short header and impossible sizes are HS_EINVAL."""
+    import ctypes as C
+    import struct
+    import hybridsmoothers._lib as L
+    h = C.c_void_p()
+    assert L.lib.hs_l1gs_load(str(tmp_path / "missing.bin").encode(), None, C.byref(h)) == L.HS_EINVAL
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"NOTL1GS0" + bytes(200))
+    assert L.lib.hs_l1gs_load(str(bad).encode(), None, C.byref(h)) == L.HS_EINVAL
+    assert b"magic" in L.lib.hs_last_error()
+    short = tmp_path / "short.bin"
+    short.write_bytes(b"HSL1GS01" + bytes(10))
+    assert L.lib.hs_l1gs_load(str(short).encode(), None, C.byref(h)) == L.HS_EINVAL
+    # a header whose array sizes do not match its partitioning
+    hdr = struct.pack("<8s6q8i7qQ", b"HSL1GS01", 100, 10, 10, 1, 10, 450,
+                      2, 0, 0, 0, 1, 1, 2, 0, 800, 0, 999, 0, 0, 0, 0, 0)
+    wrong = tmp_path / "wrong.bin"
+    wrong.write_bytes(hdr)
+    assert L.lib.hs_l1gs_load(str(wrong).encode(), None, C.byref(h)) == L.HS_EINVAL
+    assert b"sizes" in L.lib.hs_last_error()
+    assert not h.value

@@ -227,3 +227,39 @@ def test_p64_symmetric_kernel_variants(gpu, precision):
     if precision == "fp64":
         np.testing.assert_array_equal(P.sweep.lop.L.SLbuffer, ref.SL)
         np.testing.assert_array_equal(P.sweep.uop.U.SUbuffer, ref.SU)
+
+
+@pytest.mark.parametrize("case", ["sym64_fp64", "sym32_fp32", "fwd16_bf16", "bwd_view_nonsym"])
+def test_save_load_roundtrip(gpu, tmp_path, case):
+    """A saved preconditioner reloads without the matrix and applies bit for
+    bit as the original; getters agree; a corrupted payload is rejected."""
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    builder = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=20)
+    if case == "sym64_fp64":
+        A, kw = pb.poisson3d(12), dict(partsize=64, isSymA=True, sweep=hs.SymmetricSweep())
+    elif case == "sym32_fp32":
+        A, kw = pb.lognormal2d(30, seed=4), dict(partsize=32, isSymA=True, precision="fp32")
+    elif case == "fwd16_bf16":
+        A, kw = pb.lognormal3d(9, seed=5), dict(partsize=16, isSymA=True,
+                                                sweep=hs.ForwardSweep(), precision="bf16")
+    else:
+        A, kw = pb.convdiff2d(24), dict(partsize=8, isSymA=False, sweep=hs.BackwardSweep(),
+                                        cache_strategy=hs.MatrixViewCache())
+    P = builder(A, kw.pop("partsize"), **kw)
+    path = tmp_path / f"{case}.hsl1gs"
+    P.save(path)
+    Q = hs.load_preconditioner(path)
+    x = pb.uniform_rhs(A.shape[0], 3)
+    np.testing.assert_array_equal(hs.backslash(Q, x), hs.backslash(P, x))
+    assert Q.partitioning.nparts == P.partitioning.nparts
+    assert Q.precision == P.precision and Q.sweep_kind == P.sweep_kind
+    np.testing.assert_array_equal(Q._get(0), P._get(0))
+    if not isinstance(kw.get("cache_strategy"), hs.MatrixViewCache):
+        what = 2 if isinstance(kw.get("sweep"), hs.BackwardSweep) else 1
+        np.testing.assert_array_equal(Q._get(what), P._get(what))
+    raw = bytearray(path.read_bytes())
+    raw[-3] ^= 0x40
+    path.write_bytes(bytes(raw))
+    with pytest.raises(hs.HSError, match="checksum"):
+        hs.load_preconditioner(path)
