@@ -285,7 +285,10 @@ def main():
     n = A.shape[0]
     builder = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=1024)
     log("[bench] L1GS build (P=%d, symmetric, packed FP64)" % P)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter()
     prec = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep())
+    t_build = time.perf_counter() - t_build
     S = prec.packed_size
     x = torch.tensor(pb.uniform_rhs(n, 0), device="cuda")
     y = torch.empty_like(x)
@@ -324,13 +327,16 @@ def main():
         x3 = torch.tensor(pb.uniform_rhs(n3, 1), device="cuda")
         y3 = torch.empty_like(x3)
         for sw in (hs.ForwardSweep(), hs.BackwardSweep()):
+            tb = time.perf_counter()
             p3 = builder(A3, 32, isSymA=False, sweep=sw)
+            tb = time.perf_counter() - tb
             m = time_kernel(lambda: hs.ldiv_(y3, p3, x3, sync=False), args.steps, args.warmup,
                             stream)
             b = p3.packed_size * 8 + 3 * n3 * 8
             variants.append({"name": f"{args.grid2d}^2 convdiff P=32 {type(sw).__name__}",
                              "ms": round(m, 5), "bytes": b,
-                             "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
+                             "GB/s": round(b / (m * 1e-3) / 1e9, 1),
+                             "build_ms": round(tb * 1e3, 1)})
             del p3
     out = {
         "metric": "L1GS sweep GB/s (% HBM peak) at 128³; batched PCG solves/s + p99 at rtol 1e-8",
@@ -355,6 +361,7 @@ def main():
                      "algorithmic_bytes": nbytes},
         "sweeps": variants,
         "kernel_variant_ms": kernel_variants,
+        "build_ms": round(t_build * 1e3, 1),
         "kernel_variant": int(best),
     }
     if not args.quick:
