@@ -36,8 +36,8 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def sweep_bytes(n, S, tri_bytes=8, vec_bytes=8, d_bytes=8):
-    return 2 * S * tri_bytes + n * (vec_bytes + d_bytes + vec_bytes)
+def sweep_bytes(n, S, tri_bytes=8, vec_bytes=8, d_bytes=8, tris=2):
+    return tris * S * tri_bytes + n * (vec_bytes + d_bytes + vec_bytes)
 
 
 def measured_traffic(variant):
@@ -164,16 +164,20 @@ def mixed_precision_bench(hs, pb, builder, n3=192, log_fn=log):
     M = hs.DeviceMatrix(A)
     b = torch.tensor(pb.uniform_rhs(n, 2), device="cuda")
     rows = []
-    for prec in ("fp64", "fp32", "bf16"):
-        log_fn(f"[bench] mixed precision: {prec} smoother")
-        P = builder(A, 64, isSymA=True, sweep=hs.SymmetricSweep(), precision=prec)
-        rows.append(pcg_bench(hs, M, b, P, f"L1GS P=64 symmetric, {prec.upper()} smoother",
-                              reps=1))
-        del P
+    for cache, tag in ((hs.PackedBufferCache(), ""), (hs.PackedSymmetricCache(), " one-triangle")):
+        for prec in ("fp64", "fp32", "bf16"):
+            log_fn(f"[bench] mixed precision: {prec}{tag} smoother")
+            P = builder(A, 64, isSymA=True, sweep=hs.SymmetricSweep(), precision=prec,
+                        cache_strategy=cache)
+            rows.append(pcg_bench(hs, M, b, P,
+                                  f"L1GS P=64 symmetric{tag}, {prec.upper()} smoother", reps=1))
+            del P
     base = rows[0]
-    for r in rows:
+    for k, r in enumerate(rows):
+        same = rows[3 * (k // 3)]  # the FP64 solve with the same triangle storage
         r["iteration_delta"] = r["iters"] - base["iters"]
         r["speedup_vs_fp64"] = round(r["solves_per_s"] / base["solves_per_s"], 3)
+        r["speedup_vs_fp64_same_storage"] = round(r["solves_per_s"] / same["solves_per_s"], 3)
     return {"workload": f"3D 7-point lognormal-coefficient Poisson {n3}^3 (N={n}, "
                         f"nnz={A.nnz}), seed 2; FP64 CG rtol 1e-8", "solves": rows}
 
@@ -320,6 +324,17 @@ def main():
             variants.append({"name": f"128^3 P={P} symmetric {pname}", "ms": round(m, 5),
                              "bytes": b, "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
             del pv
+        # one stored triangle (U = L^T): same results, half the packed bytes
+        for pname, tb, db in (("fp64", 8, 8), ("fp32", 4, 4), ("bf16", 2, 4)):
+            pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname,
+                         cache_strategy=hs.PackedSymmetricCache())
+            m = time_kernel(lambda: hs.ldiv_(y, pv, x, sync=False), args.steps, args.warmup,
+                            stream)
+            b = sweep_bytes(n, S, tri_bytes=tb, d_bytes=db, tris=1)
+            variants.append({"name": f"128^3 P={P} symmetric {pname} one-triangle",
+                             "ms": round(m, 5), "bytes": b,
+                             "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
+            del pv
         # non-symmetric 2D upwind convection-diffusion 2048^2, P=32, DiagonalIndices path
         log("[bench] building 2048^2 convection-diffusion")
         A3 = pb.convdiff2d(args.grid2d)
@@ -374,14 +389,23 @@ def main():
             pv.set_variant(int(best))
             pcg.append(pcg_bench(hs, M, bvec, pv, f"L1GS P=64 symmetric, {pname.upper()} smoother"))
             del pv
+        for pname in ("fp64", "fp32", "bf16"):
+            pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname,
+                         cache_strategy=hs.PackedSymmetricCache())
+            pcg.append(pcg_bench(hs, M, bvec, pv,
+                                 f"L1GS P=64 symmetric one-triangle, {pname.upper()} smoother"))
+            del pv
         c4 = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend()), A)
         pcg.append(pcg_bench(hs, M, bvec, c4, "Chebyshev 4th kind degree 3, FP64"))
         c4f = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend(),
                                                     precision="fp32"), A)
         pcg.append(pcg_bench(hs, M, bvec, c4f, "Chebyshev 4th kind degree 3, FP32"))
         base = pcg[0]["solves_per_s"]
+        base1 = next(r["solves_per_s"] for r in pcg if "one-triangle, FP64" in r["name"])
         for r in pcg:
             r["speedup_vs_fp64_l1gs"] = round(r["solves_per_s"] / base, 3)
+            if "one-triangle" in r["name"]:
+                r["speedup_vs_fp64_one_triangle"] = round(r["solves_per_s"] / base1, 3)
         out["pcg_128"] = pcg
         del M
         if not args.no_mixed:

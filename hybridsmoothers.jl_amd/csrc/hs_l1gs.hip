@@ -21,6 +21,9 @@
 // with lb = block start and ub = lb + Pa*(Pa-1)/2 (symmetric) or lb (backward);
 // Pa = rows of this partition (only the last partition can be shorter).
 // The reference row-major layout is produced on request by hs_l1gs_get.
+// The one-triangle cache (HS_CACHE_PACKED_SYM, symmetric A) stores only the
+// lower triangle, k*P*(P-1)/2 per block like a forward sweep, and reads
+// U(i,j) = L(j,i) back from an LDS copy made during the forward pass.
 //
 // Recurrence.  Lane i of a segment owns row i.  Forward step j: every lane
 // forms (x_i - acc_i)/d_i, the value of lane j is broadcast, and lanes i > j
@@ -402,6 +405,35 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
     (void)sweep_seg_body<PM, MODE, TS, TC, false>(x, y, T, D, n, P, nparts, k, lane, f);
 }
 
+// One-triangle symmetric sweep (HS_CACHE_PACKED_SYM): one wave per workgroup,
+// 64 / PM partitions per wave, each with its LDS copy of L (symt_lds(PM)
+// elements: 17 KB per wave in FP64 at PM = 64), so that LDS does not limit
+// the waves per CU below what the recurrence's latency needs.
+template <int PM, class TS, class TC, bool CG = false>
+__global__ __launch_bounds__(64) void k_sweep_symt(const double *x, double *y,
+                                                   const TS *__restrict__ T,
+                                                   const TC *__restrict__ D, int64_t n, int P,
+                                                   int64_t nparts, const int *__restrict__ done,
+                                                   CGFuse f = CGFuse{}) {
+    if (done && *done) return;
+    constexpr int SEGS = 64 / PM;
+    __shared__ TS lds[SEGS * symt_lds(PM)];
+    const int lane = threadIdx.x % PM;
+    const int seg = threadIdx.x / PM;
+    const int64_t k = (int64_t)blockIdx.x * SEGS + seg;
+    if (__all(k >= nparts)) {
+        if constexpr (CG)
+            if (threadIdx.x == 0) f.partials[blockIdx.x] = 0.0;
+        return;
+    }
+    double v = sweep_seg_body<PM, MODE_SYMT, TS, TC, CG>(x, y, T, D, n, P, nparts, k, lane, f,
+                                                         lds + seg * symt_lds(PM));
+    if constexpr (CG) {
+        v = wave_sum(v);  // r'z partial of this workgroup
+        if (threadIdx.x == 0) f.partials[blockIdx.x] = v;
+    }
+}
+
 // Partitions larger than a wave: one thread per partition, rows in sequence,
 // summing in the reference's order (left-looking).  Correctness path for
 // large partsize (e.g. partsize = N in the Gauss-Seidel sanity checks).
@@ -528,6 +560,12 @@ struct L1GSPlan {
     Staging stage;
 
     size_t elem() const { return prec == HS_PREC_FP64 ? 8 : (prec == HS_PREC_FP32 ? 4 : 2); }
+    // block layout of the packed buffer (block_bases) and its element count
+    int tri_mode() const { return cache == HS_CACHE_PACKED_SYM ? (int)MODE_SYMT : sweep; }
+    int64_t tri_elems() const {
+        if (cache == HS_CACHE_VIEW) return 0;
+        return (cache == HS_CACHE_PACKED && sweep == HS_SWEEP_SYMMETRIC) ? 2 * S : S;
+    }
 };
 
 template <int PM, class TS, class TC>
@@ -553,9 +591,36 @@ static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *
     }
 }
 
+template <int PM, class TS, class TC, bool CG>
+static unsigned launch_symt_pm(const L1GSPlan &h, const double *x, double *y, const TC *D,
+                               hipStream_t st, const int *done, const CGFuse &f) {
+    constexpr int SEGS = 64 / PM;
+    const unsigned grid = grid_for(h.nparts, SEGS);
+    hipLaunchKernelGGL((k_sweep_symt<PM, TS, TC, CG>), dim3(grid), dim3(64), 0, st, x, y,
+                       reinterpret_cast<const TS *>(h.T.get()), D, h.n, (int)h.P, h.nparts, done,
+                       f);
+    return grid;
+}
+
+// HS_CACHE_PACKED_SYM (partsize <= 64, checked at build); returns the number
+// of workgroups (= r'z partials in the CG form)
+template <class TS, class TC, bool CG>
+static unsigned launch_symt(const L1GSPlan &h, const double *x, double *y, const TC *D,
+                            hipStream_t st, const int *done, const CGFuse &f) {
+    if (h.P <= 4) return launch_symt_pm<4, TS, TC, CG>(h, x, y, D, st, done, f);
+    if (h.P <= 8) return launch_symt_pm<8, TS, TC, CG>(h, x, y, D, st, done, f);
+    if (h.P <= 16) return launch_symt_pm<16, TS, TC, CG>(h, x, y, D, st, done, f);
+    if (h.P <= 32) return launch_symt_pm<32, TS, TC, CG>(h, x, y, D, st, done, f);
+    return launch_symt_pm<64, TS, TC, CG>(h, x, y, D, st, done, f);
+}
+
 template <class TS, class TC>
 static void launch_packed(const L1GSPlan &h, const double *x, double *y, const TC *D,
                           hipStream_t st, const int *done) {
+    if (h.cache == HS_CACHE_PACKED_SYM) {
+        launch_symt<TS, TC, false>(h, x, y, D, st, done, CGFuse{});
+        return;
+    }
     if (h.P == 64 && h.sweep == HS_SWEEP_SYMMETRIC && h.variant > 0) {
         const TS *T = reinterpret_cast<const TS *>(h.T.get());
         const dim3 grid(grid_for(h.nparts, 1)), block(64);
@@ -603,6 +668,8 @@ static unsigned launch_fused(const L1GSPlan &h, double *z, const TC *D, hipStrea
                              const CGFuse &f) {
     const int *done = reinterpret_cast<const int *>(reinterpret_cast<const char *>(f.st) +
                                                     offsetof(CGDev, done));
+    if (h.cache == HS_CACHE_PACKED_SYM)
+        return launch_symt<TS, TC, true>(h, (const double *)nullptr, z, D, st, done, f);
     if (h.P == 64 && h.variant > 0) {
         const TS *T = reinterpret_cast<const TS *>(h.T.get());
         const dim3 grid(grid_for(h.nparts, 1)), block(64);
@@ -622,7 +689,8 @@ static unsigned launch_fused(const L1GSPlan &h, double *z, const TC *D, hipStrea
 }
 
 bool l1gs_can_fuse(const L1GSPlan &h) {
-    return h.cache == HS_CACHE_PACKED && h.sweep == HS_SWEEP_SYMMETRIC && h.P <= 64 && h.n > 0;
+    return (h.cache == HS_CACHE_PACKED || h.cache == HS_CACHE_PACKED_SYM) &&
+           h.sweep == HS_SWEEP_SYMMETRIC && h.P <= 64 && h.n > 0;
 }
 
 unsigned l1gs_apply_fused(const L1GSPlan &h, double *z, hipStream_t st, const CGFuse &f) {
@@ -708,8 +776,9 @@ static void build_packed(L1GSPlan &h, const DBuf<int64_t> &ptr, const DBuf<int64
                          const DBuf<double> &val, const int64_t *h_ptr, const int64_t *h_ind,
                          const double *h_val, double eta, hipStream_t st) {
     const int want_L = h.sweep != HS_SWEEP_BACKWARD;
-    const int want_U = h.sweep != HS_SWEEP_FORWARD;
-    const int64_t elems = h.S * (h.sweep == HS_SWEEP_SYMMETRIC ? 2 : 1);
+    const int want_U = h.sweep != HS_SWEEP_FORWARD && h.cache != HS_CACHE_PACKED_SYM;
+    const int mode = h.tri_mode();
+    const int64_t elems = h.tri_elems();
     h.T.alloc(elems * sizeof(TS));
     if (elems) HS_CHECK_HIP(hipMemsetAsync(h.T.get(), 0, h.T.bytes(), st));
     TS *T = reinterpret_cast<TS *>(h.T.get());
@@ -718,13 +787,13 @@ static void build_packed(L1GSPlan &h, const DBuf<int64_t> &ptr, const DBuf<int64
         // one pass: D_Dl1 and both triangles (the reference launches its
         // packing kernel once per triangle and recomputes D_Dl1, :805-837)
         hipLaunchKernelGGL((k_setup_rows<TS>), dim3(grid), dim3(256), 0, st, h.n, h.P, ptr.get(),
-                           ind.get(), val.get(), eta, h.D.get(), T, h.sweep, want_L, want_U);
+                           ind.get(), val.get(), eta, h.D.get(), T, mode, want_L, want_U);
         check_launch();
         return;
     }
     nonsym_D(h, h_ptr, h_ind, h_val, eta, st);
     hipLaunchKernelGGL((k_pack_cols<TS>), dim3(grid), dim3(256), 0, st, h.n, h.P, ptr.get(),
-                       ind.get(), val.get(), T, h.sweep, want_L, want_U);
+                       ind.get(), val.get(), T, mode, want_L, want_U);
     check_launch();
 }
 
@@ -748,8 +817,14 @@ int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const doubl
         HS_REQUIRE(n >= 0, HS_EINVAL, "matrix size must be non-negative");
         HS_REQUIRE(fmt == HS_FMT_CSC || fmt == HS_FMT_CSR, HS_EINVAL, "unknown matrix format");
         HS_REQUIRE(sweep >= 0 && sweep <= 2, HS_EINVAL, "unknown sweep");
-        HS_REQUIRE(cache == HS_CACHE_PACKED || cache == HS_CACHE_VIEW, HS_EINVAL,
-                   "unknown cache strategy");
+        HS_REQUIRE(cache == HS_CACHE_PACKED || cache == HS_CACHE_VIEW ||
+                       cache == HS_CACHE_PACKED_SYM,
+                   HS_EINVAL, "unknown cache strategy");
+        HS_REQUIRE(cache != HS_CACHE_PACKED_SYM ||
+                       (sweep == HS_SWEEP_SYMMETRIC && is_sym && partsize <= 64),
+                   HS_EUNSUPPORTED,
+                   "the one-triangle packed cache needs isSymA = true, SymmetricSweep and "
+                   "partsize <= 64");
         HS_REQUIRE(precision >= HS_PREC_FP64 && precision <= HS_PREC_BF16, HS_EINVAL,
                    "unknown precision");
         HS_REQUIRE(!(cache == HS_CACHE_VIEW && precision != HS_PREC_FP64), HS_EUNSUPPORTED,
@@ -805,7 +880,7 @@ int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const doubl
         }
         h->D.alloc(n);
         if (n > 0) {
-            if (cache == HS_CACHE_PACKED) {
+            if (cache != HS_CACHE_VIEW) {
                 if (precision == HS_PREC_FP64)
                     build_packed<double>(*h, dptr, dind, dval, ptr, ind, val, eta, st);
                 else if (precision == HS_PREC_FP32)
@@ -884,14 +959,15 @@ int hs_l1gs_get(hs_l1gs_t h, int what, double *out) {
             return;
         }
         HS_REQUIRE(what == HS_GET_SL || what == HS_GET_SU, HS_EINVAL, "unknown field");
-        HS_REQUIRE(h->cache == HS_CACHE_PACKED, HS_EINVAL,
+        HS_REQUIRE(h->cache != HS_CACHE_VIEW, HS_EINVAL,
                    "packed buffers exist only for PackedBufferCache");
         const bool has = (what == HS_GET_SL) ? h->sweep != HS_SWEEP_BACKWARD
                                              : h->sweep != HS_SWEEP_FORWARD;
         HS_REQUIRE(has, HS_EINVAL,
                    what == HS_GET_SL ? "this sweep has no lower buffer"
                                      : "this sweep has no upper buffer");
-        const int64_t elems = h->S * (h->sweep == HS_SWEEP_SYMMETRIC ? 2 : 1);
+        const int64_t elems = h->tri_elems();
+        const bool one_tri = h->cache == HS_CACHE_PACKED_SYM;
         std::vector<double> internal(elems);
         const size_t es = h->elem();
         std::vector<uint8_t> raw(elems * es);
@@ -920,15 +996,17 @@ int hs_l1gs_get(hs_l1gs_t h, int what, double *out) {
             int64_t Pa = (n - s < P) ? n - s : P;
             int64_t boff = k * ((P * (P - 1)) / 2);
             int64_t lb, ub;
-            block_bases(k, P, Pa, h->sweep, lb, ub);
+            block_bases(k, P, Pa, h->tri_mode(), lb, ub);
             for (int64_t i = 0; i < Pa; ++i) {
                 if (what == HS_GET_SL) {
                     for (int64_t j = 0; j < i; ++j)
                         out[boff + (i * (i - 1)) / 2 + j] = internal[lb + colL_off(j, Pa) + (i - j - 1)];
                 } else {
                     int64_t row_start = i * Pa - (i * (i + 1)) / 2;
-                    for (int64_t j = i + 1; j < Pa; ++j)
-                        out[boff + row_start + (j - i - 1)] = internal[ub + colU_off(j, Pa) + i];
+                    for (int64_t j = i + 1; j < Pa; ++j)  // U(i,j) = L(j,i) with one triangle
+                        out[boff + row_start + (j - i - 1)] =
+                            one_tri ? internal[lb + colL_off(i, Pa) + (j - i - 1)]
+                                    : internal[ub + colU_off(j, Pa) + i];
                 }
             }
         }
@@ -1032,15 +1110,18 @@ int hs_l1gs_load(const char *path, void *stream, hs_l1gs_t *out) {
                    "not a saved L1GS preconditioner (bad magic)");
         const int64_t n = hd.n;
         HS_REQUIRE(n >= 0 && hd.P > 0 && hd.sweep >= 0 && hd.sweep <= 2 &&
-                       (hd.cache == HS_CACHE_PACKED || hd.cache == HS_CACHE_VIEW) &&
+                       (hd.cache == HS_CACHE_PACKED || hd.cache == HS_CACHE_VIEW ||
+                        (hd.cache == HS_CACHE_PACKED_SYM && hd.sweep == HS_SWEEP_SYMMETRIC &&
+                         hd.P <= 64)) &&
                        hd.prec >= HS_PREC_FP64 && hd.prec <= HS_PREC_BF16 && hd.variant >= 0 &&
                        hd.variant <= 2,
                    HS_EINVAL, "corrupt L1GS file (header fields)");
         const int64_t nparts = n > 0 ? (n + hd.P - 1) / hd.P : 0;
         const int64_t elem = hd.prec == HS_PREC_FP64 ? 8 : (hd.prec == HS_PREC_FP32 ? 4 : 2);
-        const int64_t tri_elems = hd.cache == HS_CACHE_PACKED
-                                      ? (hd.sweep == HS_SWEEP_SYMMETRIC ? 2 * hd.S : hd.S)
-                                      : 0;
+        const int64_t tri_elems =
+            hd.cache == HS_CACHE_VIEW
+                ? 0
+                : (hd.cache == HS_CACHE_PACKED && hd.sweep == HS_SWEEP_SYMMETRIC ? 2 * hd.S : hd.S);
         HS_REQUIRE(hd.nparts == nparts && hd.nbytes[0] == n * 8 &&
                        hd.nbytes[1] == (hd.prec == HS_PREC_FP64 ? 0 : n * 4) &&
                        hd.nbytes[2] == tri_elems * elem,

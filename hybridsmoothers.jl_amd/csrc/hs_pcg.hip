@@ -100,11 +100,12 @@ __global__ __launch_bounds__(256) void k_cg_spmv_fused(int64_t n, SellView<doubl
     const double beta = first ? 0.0 : st->beta;
     const double *pold = (k & 1) ? pa : pb;
     double *pnew = (k & 1) ? pb : pa;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    int64_t i = (int64_t)tile * blockDim.x + threadIdx.x;
     double d = 0.0;
     if (i < n) d = cg_spmv_row(A, z, pold, pnew, q, first, beta, i);
     double bs = block_sum(d, sm);
-    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+    if (threadIdx.x == 0) partials[tile] = bs;
 }
 
 // x += alpha p; r -= alpha q

@@ -145,10 +145,33 @@ __device__ __forceinline__ double block_sum_wide(double v, double *smem16) {
 
 constexpr int FIN_BLOCK = 1024;  // threads of the finalize kernels
 
+// The dispatcher hands workgroup b of a launch to XCD b mod 8.  xcd_tile(b, G)
+// numbers the G tiles so that each XCD works through one contiguous range of
+// them (XCD x gets tiles [x q + min(x, r), ...) with q = G / 8, r = G mod 8):
+// a stencil row's neighbours in the next and previous planes were then loaded
+// by the same XCD, into the same L2.  A bijection of [0, G).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned G) {
+    const unsigned x = b & 7u, q = G >> 3, r = G & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// Thread t adds partials t, t + blockDim, t + 2 blockDim, ... one after the
+// other; the loads of 8 consecutive terms are issued before their additions
+// (same order, same result), so the loop is not one load latency per term.
 __device__ __forceinline__ double finalize_sum(const double *__restrict__ partials, unsigned count,
                                               unsigned stride, double *smem16) {
+    constexpr unsigned U = 8;
+    const unsigned step = blockDim.x;
     double v = 0.0;
-    for (unsigned b = threadIdx.x; b < count; b += blockDim.x) v += partials[(size_t)b * stride];
+    unsigned b = threadIdx.x;
+    for (; b + (U - 1) * step < count; b += U * step) {
+        double t[U];
+#pragma unroll
+        for (unsigned k = 0; k < U; ++k) t[k] = partials[(size_t)(b + k * step) * stride];
+#pragma unroll
+        for (unsigned k = 0; k < U; ++k) v += t[k];
+    }
+    for (; b < count; b += step) v += partials[(size_t)b * stride];
     return block_sum_wide(v, smem16);  // valid in thread 0
 }
 

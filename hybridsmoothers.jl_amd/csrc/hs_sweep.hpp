@@ -40,8 +40,9 @@ __device__ __forceinline__ void static_for(F &&f) {
 }
 
 // elements of a segment's LDS copy of L in the one-triangle symmetric sweep:
-// PM rows of PM + 1 (see sweep_full_symt)
-__host__ __device__ constexpr int symt_lds(int PM) { return PM * (PM + 1); }
+// PM rows of PM + 1 (sweep_full_symt); for PM = 64 the half-size copy of
+// sweep_full_symt64, 32 lines of 65 and a 64-element area for dead stores
+__host__ __device__ constexpr int symt_lds(int PM) { return PM == 64 ? 32 * 65 + 64 : PM * (PM + 1); }
 // start of the lower and upper triangles of partition k (Pa rows)
 __host__ __device__ __forceinline__ void block_bases(int64_t k, int64_t P, int64_t Pa, int mode,
                                                      int64_t &lb, int64_t &ub) {
@@ -401,6 +402,71 @@ __device__ __forceinline__ TC sweep_full_symt(TC xr, TC dr, TC rr, int lane,
     return (lane == 0) ? q : yfin;
 }
 
+// PM = 64 form of sweep_full_symt with half the LDS (2144 elements instead of
+// 4160), so that an FP64 wave needs 17 KB.  Row r of L has r entries; rows r
+// and 63 - r share line m = min(r, 63 - r) of 65 elements: row m (m < 32) at
+// positions [0, m), row 63 - m at [m + 1, 64).  Lane r stores L(r, j) at
+// wbase + j with wbase = 65 r (r < 32) or 65 (63 - r) + 64 - r (r >= 32).  A
+// lane whose row is complete (lane j from step j on) would store into the
+// other row of its line, so at step j lane j's base moves (v_writelane, one
+// operation per step) to G - j, G = 32 * 65: the dead stores of step s then go
+// to G + s - r, distinct for the dead lanes r <= s and inside [G, G + 63).
+// The backward pass reads row j at a step-constant offset from lds + lane:
+// 65 j (j < 32) or 65 (63 - j) + 64 - j (j >= 32).  Same loads, products and
+// order as sweep_full_symt: bit-identical results.
+template <int RING, class TS, class TC>
+__device__ __forceinline__ TC sweep_full_symt64(TC xr, TC dr, TC rr, int lane,
+                                                const TS *__restrict__ Lk, TS *lds) {
+    constexpr int PM = 64, NL = PM - 1;
+    constexpr int R = RING < NL ? RING : NL;
+    constexpr int RS = 65, G = 32 * RS;
+    auto load_col = [&](int j) -> TS {
+        const int off = (j == 0) ? (lane > 0 ? lane - 1 : 0) : lane - j - 1;
+        return Lk[colL_off(j, PM) + off];
+    };
+    int wbase = lane < 32 ? lane * RS : (63 - lane) * RS + 64 - lane;
+    asm volatile("" : "+v"(wbase));
+    const TS *const rrow = lds + lane;
+    const int base4 = seg_base4(PM);
+    TS ring[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) ring[c] = load_col(c);
+    TC acc = (TC)0, yfin = (TC)0;
+    static_for<0, NL>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        __builtin_amdgcn_sched_barrier(0);
+        const TS v = ring[j % R];
+        if constexpr (j + R < NL) ring[j % R] = load_col(j + R);
+        wbase = hs_writelane_i32(G - j, j, wbase);
+        lds[wbase + j] = v;
+        TC q = quot_fast(xr - acc, dr, rr);
+        TC yj = bcast_step<PM, j>(q, yfin, base4, lane);
+        acc = acc + load_as<TC>(v) * yj;  // dead for lanes <= j
+    });
+    TC q = quot_fast(xr - acc, dr, rr);
+    TC yr = (lane == PM - 1) ? q : yfin;
+    const TC xb = dr * yr;  // y .= D_Dl1 .* y  (:897)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    auto row_at = [](int j) { return j < 32 ? j * RS : (63 - j) * RS + 64 - j; };
+    constexpr int RB = 4;
+    TS rb[RB];
+#pragma unroll
+    for (int c = 0; c < RB; ++c) rb[c] = rrow[row_at(PM - 1 - c)];
+    acc = (TC)0;
+    yfin = (TC)0;
+    static_for<0, NL>([&](auto JJ) {
+        constexpr int jj = decltype(JJ)::value;
+        constexpr int j = PM - 1 - jj;
+        const TS v = rb[jj % RB];
+        if constexpr (jj + RB < NL) rb[jj % RB] = rrow[row_at(PM - 1 - (jj + RB))];
+        TC qb = quot_fast(xb - acc, dr, rr);
+        TC yj = bcast_step<PM, j>(qb, yfin, base4, lane);
+        acc = acc + load_as<TC>(v) * yj;  // dead for lanes >= j
+    });
+    q = quot_fast(xb - acc, dr, rr);
+    return (lane == 0) ? q : yfin;
+}
+
 // Row prologue of the fused CG form: r -= alpha q; x += alpha p (the k_cg_axpy
 // arithmetic); returns the updated r, the sweep's input.
 __device__ __forceinline__ double cg_prologue(const CGFuse &f, int64_t i) {
@@ -453,7 +519,9 @@ __device__ __forceinline__ double sweep_seg_run(In &&input, double *y, const TS 
     if (__any(!safe_divisor(dr))) {
         yr = sweep_generic<PM, MODE, false, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
     } else if (__all(Pa == PM)) {
-        if constexpr (MODE == MODE_SYMT)
+        if constexpr (MODE == MODE_SYMT && PM == 64)
+            yr = sweep_full_symt64<32, TS, TC>(xr, dr, rr, lane, Lk, lds);
+        else if constexpr (MODE == MODE_SYMT)
             yr = sweep_full_symt<PM, 32, TS, TC>(xr, dr, rr, lane, Lk, lds);
         else
             yr = sweep_full<PM, MODE, 16, TS, TC>(xr, dr, rr, lane, Lk, Uk);

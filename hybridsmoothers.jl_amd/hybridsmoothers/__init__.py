@@ -11,7 +11,8 @@ from .chebyshev import (  # noqa: F401
 from .chebyshev import ldiv_ as cheb_ldiv_  # noqa: F401
 from .l1gs import (  # noqa: F401
     CPU, GPU, BackwardSweep, BlockPartitioning, CPUConfig, ForwardSweep, GPUConfig,
-    L1GSPrecBuilder, L1GSPreconditioner, MatrixViewCache, PackedBufferCache, ROCBackend,
+    L1GSPrecBuilder, L1GSPreconditioner, MatrixViewCache, PackedBufferCache,
+    PackedSymmetricCache, ROCBackend,
     Symmetric, SymmetricSweep, backslash, functional,
 )
 from .l1gs import ldiv_ as _l1gs_ldiv_

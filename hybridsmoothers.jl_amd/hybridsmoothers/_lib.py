@@ -25,7 +25,7 @@ HS_EAGAIN = -7
 
 FMT_CSC, FMT_CSR = 0, 1
 SWEEP_FORWARD, SWEEP_BACKWARD, SWEEP_SYMMETRIC = 0, 1, 2
-CACHE_PACKED, CACHE_VIEW = 0, 1
+CACHE_PACKED, CACHE_VIEW, CACHE_PACKED_SYM = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 LOC_HOST, LOC_DEVICE = 0, 1
 FLAG_ASYNC = 1

@@ -79,6 +79,14 @@ class MatrixViewCache(AbstractCacheStrategy):
     code = L.CACHE_VIEW
 
 
+class PackedSymmetricCache(AbstractCacheStrategy):
+    """Packed lower triangle only, U = L^T inside each partition: for a
+    symmetric A (isSymA=True) with SymmetricSweep and partsize <= 64.  Not in
+    the reference, whose PackedBufferCache stores both triangles (:788-799);
+    the sweep gives the same results from half the packed bytes."""
+    code = L.CACHE_PACKED_SYM
+
+
 class Backend:
     def __repr__(self):
         return f"{type(self).__name__}()"
@@ -352,7 +360,8 @@ class L1GSPreconditioner:
 
 _SWEEPS = {L.SWEEP_FORWARD: ForwardSweep, L.SWEEP_BACKWARD: BackwardSweep,
            L.SWEEP_SYMMETRIC: SymmetricSweep}
-_CACHES = {L.CACHE_PACKED: PackedBufferCache, L.CACHE_VIEW: MatrixViewCache}
+_CACHES = {L.CACHE_PACKED: PackedBufferCache, L.CACHE_VIEW: MatrixViewCache,
+           L.CACHE_PACKED_SYM: PackedSymmetricCache}
 
 
 def load(path, backend=None, stream=None) -> L1GSPreconditioner:

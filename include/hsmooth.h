@@ -54,6 +54,10 @@ extern "C" {
 #define HS_SWEEP_SYMMETRIC 2
 #define HS_CACHE_PACKED 0      /* PackedBufferCache */
 #define HS_CACHE_VIEW 1        /* MatrixViewCache */
+#define HS_CACHE_PACKED_SYM 2  /* packed, lower triangle only, U = L^T: symmetric A (isSymA or
+                                  a symmetric CSR), SymmetricSweep, partsize <= 64; no
+                                  counterpart in the reference (its PackedBufferCache stores
+                                  both triangles, :788-799); same results, half the bytes */
 #define HS_PREC_FP64 0         /* FP64 values, FP64 arithmetic */
 #define HS_PREC_FP32 1         /* FP32 values and packed buffers, FP32 accumulation */
 #define HS_PREC_BF16 2         /* BF16 off-diagonals, FP32 diagonal and accumulation */

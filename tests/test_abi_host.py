@@ -47,13 +47,13 @@ def test_library_is_gfx950_only():
     assert targets == {b"gfx950"}
 
 
-def _build_args(n=9, partsize=2, threads=4, blocks=4, sweep=2, cache=0, prec=0, fmt=0):
+def _build_args(n=9, partsize=2, threads=4, blocks=4, sweep=2, cache=0, prec=0, fmt=0, sym=1):
     from hybridsmoothers import problems as pb
     A = pb.poisson_test_matrix(n)
     ptr = A.indptr.astype(np.int64) + 1
     ind = A.indices.astype(np.int64) + 1
     val = A.data.astype(np.float64)
-    return (n, ptr, ind, val, fmt, 1, partsize, 1.5, sweep, cache, prec, threads, blocks)
+    return (n, ptr, ind, val, fmt, sym, partsize, 1.5, sweep, cache, prec, threads, blocks)
 
 
 def _call_build(args):
@@ -85,6 +85,17 @@ def test_matrix_view_precision_unsupported():
     from hybridsmoothers import _lib as L
     st, err = _call_build(_build_args(cache=1, prec=1))
     assert st == L.HS_EUNSUPPORTED
+
+
+@pytest.mark.parametrize("kw", [dict(sweep=0), dict(sweep=1), dict(sym=0), dict(partsize=65),
+                                dict(sym=0, fmt=1)])
+def test_one_triangle_cache_limits(kw):
+    """The one-triangle packed cache (U = L^T) needs isSymA, SymmetricSweep and
+    partsize <= 64; anything else is refused before any device work."""
+    from hybridsmoothers import _lib as L
+    st, err = _call_build(_build_args(cache=L.CACHE_PACKED_SYM, n=130, **kw))
+    assert st == L.HS_EUNSUPPORTED
+    assert "one-triangle" in err
 
 
 def test_null_handles():
