@@ -297,7 +297,8 @@ __device__ __forceinline__ TC sweep_ring64(TC xr, TC dr, TC rr, int lane,
     lds_fence();
     static_for<0, G>([&](auto T) { col[0][decltype(T)::value] = read_at(T); });
 
-    TC acc = (TC)0, yfin = (TC)0, ysym = xr;
+    RowAcc<TC> row(xr);
+    TC yfin = (TC)0, ysym = xr;
     static_for<0, NG>([&](auto Gi) {
         constexpr int g = decltype(Gi)::value;
         constexpr int t0 = g * G;
@@ -321,21 +322,20 @@ __device__ __forceinline__ TC sweep_ring64(TC xr, TC dr, TC rr, int lane,
             if constexpr (t < NSTEP) {
                 if constexpr (t == P - 1) {
                     // last forward row, y .= D_Dl1 .* y (:897), then the backward pass
-                    TC q = quot_fast(xr - acc, dr, rr);
+                    TC q = row.q(dr, rr);
                     TC yf = (lane == P - 1) ? q : yfin;
                     ysym = dr * yf;
-                    acc = (TC)0;
+                    row = RowAcc<TC>(ysym);
                     yfin = (TC)0;
                 }
-                const TC xin = (t < P - 1) ? xr : ysym;
                 constexpr int j = step_col(t);
-                TC q = quot_fast(xin - acc, dr, rr);
+                TC q = row.q(dr, rr);
                 TC yj = bcast_keep<64>(q, j, yfin);
-                acc = acc + load_as<TC>(col[g & 1][t - t0]) * yj;  // dead for finished lanes
+                row.add(load_as<TC>(col[g & 1][t - t0]), yj);  // dead for finished lanes
             }
         });
     });
-    TC q = quot_fast(ysym - acc, dr, rr);
+    TC q = row.q(dr, rr);
     return (lane == 0) ? q : yfin;
 }
 
@@ -452,20 +452,20 @@ __global__ void k_sweep_thread(const double *x, double *y, const TS *__restrict_
         for (int64_t i = 0; i < Pa; ++i) y[s + i] = x[s + i];
     if (MODE != MODE_BWD) {
         for (int64_t i = 0; i < Pa; ++i) {
-            TC acc = 0;
+            RowAcc<TC> row((TC)y[s + i]);
             for (int64_t j = 0; j < i; ++j)
-                acc = acc + load_as<TC>(T[lb + colL_off(j, Pa) + (i - j - 1)]) * (TC)y[s + j];
-            y[s + i] = (double)(((TC)y[s + i] - acc) / D[s + i]);
+                row.add(load_as<TC>(T[lb + colL_off(j, Pa) + (i - j - 1)]), (TC)y[s + j]);
+            y[s + i] = (double)row.template q<false>(D[s + i], (TC)0);
         }
     }
     if (MODE == MODE_SYM)
         for (int64_t i = 0; i < Pa; ++i) y[s + i] = (double)(D[s + i] * (TC)y[s + i]);
     if (MODE != MODE_FWD) {
         for (int64_t i = Pa - 1; i >= 0; --i) {
-            TC acc = 0;
+            RowAcc<TC> row((TC)y[s + i]);
             for (int64_t j = i + 1; j < Pa; ++j)
-                acc = acc + load_as<TC>(T[ub + colU_off(j, Pa) + i]) * (TC)y[s + j];
-            y[s + i] = (double)(((TC)y[s + i] - acc) / D[s + i]);
+                row.add(load_as<TC>(T[ub + colU_off(j, Pa) + i]), (TC)y[s + j]);
+            y[s + i] = (double)row.template q<false>(D[s + i], (TC)0);
         }
     }
 }

@@ -218,6 +218,37 @@ __device__ __forceinline__ bool safe_divisor(float d) {
     return a >= 0x1p-60f && a <= 0x1p60f;
 }
 
+// One row of a recurrence: its input x, the products L(i,j) y_j added one by
+// one, and the quotient (x - sum) / d.  FP64 is the reference's arithmetic bit
+// for bit: the products are summed unfused in order and the quotient is
+// correctly rounded.  FP32 (the reduced-precision smoothers, which have no
+// reference arithmetic to reproduce) keeps the remainder rem = x - sum, takes
+// each term off with one FMA, and multiplies by the reciprocal of d: two VALU
+// operations per step instead of five.  FAST = false divides (unsafe d).
+template <class TC>
+struct RowAcc;
+template <>
+struct RowAcc<double> {
+    double x, acc;
+    __device__ __forceinline__ explicit RowAcc(double xr) : x(xr), acc(0.0) {}
+    template <bool FAST = true>
+    __device__ __forceinline__ double q(double d, double r) const {
+        return quot<FAST>(x - acc, d, r);
+    }
+    __device__ __forceinline__ void add(double l, double y) { acc = acc + l * y; }
+};
+template <>
+struct RowAcc<float> {
+    float rem;
+    __device__ __forceinline__ explicit RowAcc(float xr) : rem(xr) {}
+    template <bool FAST = true>
+    __device__ __forceinline__ float q(float d, float r) const {
+        if constexpr (FAST) return rem * r;
+        else return rem / d;
+    }
+    __device__ __forceinline__ void add(float l, float y) { rem = __builtin_fmaf(-l, y, rem); }
+};
+
 // Any partition shape (last, short partition; P below the segment width):
 // steps guarded by the actual size Pa.
 template <int PM, int MODE, bool FAST, class TS, class TC>
@@ -226,17 +257,18 @@ __device__ __forceinline__ TC sweep_generic(TC xr, TC dr, TC rr, int lane, int P
                                             const TS *__restrict__ Uk) {
     TC yr = xr;
     if constexpr (MODE != MODE_BWD) {
-        TC acc = (TC)0, yfin = (TC)0;
+        RowAcc<TC> row(xr);
+        TC yfin = (TC)0;
 #pragma unroll
         for (int j = 0; j < PM; ++j) {
             if (j < Pa) {
-                TC q = quot<FAST>(xr - acc, dr, rr);
+                TC q = row.template q<FAST>(dr, rr);
                 yfin = (lane == j) ? q : yfin;
                 if (j < Pa - 1) {
                     TC yj = seg_bcast<PM>(q, j);
                     bool has = (lane > j) && live;
                     TC l = has ? load_as<TC>(Lk[colL_off(j, Pa) + (lane - j - 1)]) : (TC)0;
-                    if (lane > j) acc = acc + l * yj;
+                    if (lane > j) row.add(l, yj);
                 }
             }
         }
@@ -244,12 +276,13 @@ __device__ __forceinline__ TC sweep_generic(TC xr, TC dr, TC rr, int lane, int P
     }
     if constexpr (MODE == MODE_SYM || MODE == MODE_SYMT) xr = dr * yr;  // y .= D_Dl1 .* y (:897)
     if constexpr (MODE != MODE_FWD) {
-        TC acc = (TC)0, yfin = (TC)0;
+        RowAcc<TC> row(xr);
+        TC yfin = (TC)0;
 #pragma unroll
         for (int jj = 0; jj < PM; ++jj) {
             const int j = PM - 1 - jj;
             if (j < Pa) {
-                TC q = quot<FAST>(xr - acc, dr, rr);
+                TC q = row.template q<FAST>(dr, rr);
                 yfin = (lane == j) ? q : yfin;
                 if (j > 0) {
                     TC yj = seg_bcast<PM>(q, j);
@@ -260,7 +293,7 @@ __device__ __forceinline__ TC sweep_generic(TC xr, TC dr, TC rr, int lane, int P
                                 ? load_as<TC>(Lk[colL_off(lane, Pa) + (j - lane - 1)])
                                 : load_as<TC>(Uk[colU_off(j, Pa) + lane]);
                     }
-                    if (lane < j) acc = acc + u * yj;
+                    if (lane < j) row.add(u, yj);
                 }
             }
         }
@@ -305,32 +338,34 @@ __device__ __forceinline__ TC sweep_full(TC xr, TC dr, TC rr, int lane,
 
     TC yr = xr;
     if constexpr (MODE != MODE_BWD) {
-        TC acc = (TC)0, yfin = (TC)0;
+        RowAcc<TC> row(xr);
+        TC yfin = (TC)0;
         static_for<0, PM - 1>([&](auto J) {
             constexpr int j = decltype(J)::value;
             const TS v = ring[j % R];
             if constexpr (j + R < NC) ring[j % R] = load_col(j + R);
-            TC q = quot_fast(xr - acc, dr, rr);
+            TC q = row.q(dr, rr);
             TC yj = bcast_step<PM, j>(q, yfin, base4, lane);
-            acc = acc + load_as<TC>(v) * yj;  // dead for lanes <= j
+            row.add(load_as<TC>(v), yj);  // dead for lanes <= j
         });
-        TC q = quot_fast(xr - acc, dr, rr);
+        TC q = row.q(dr, rr);
         yr = (lane == PM - 1) ? q : yfin;
     }
     if constexpr (MODE == MODE_SYM) xr = dr * yr;  // y .= D_Dl1 .* y  (:897)
     if constexpr (MODE != MODE_FWD) {
-        TC acc = (TC)0, yfin = (TC)0;
+        RowAcc<TC> row(xr);
+        TC yfin = (TC)0;
         static_for<0, PM - 1>([&](auto JJ) {
             constexpr int jj = decltype(JJ)::value;
             constexpr int j = PM - 1 - jj;
             constexpr int c = NL + jj;
             const TS v = ring[c % R];
             if constexpr (c + R < NC) ring[c % R] = load_col(c + R);
-            TC q = quot_fast(xr - acc, dr, rr);
+            TC q = row.q(dr, rr);
             TC yj = bcast_step<PM, j>(q, yfin, base4, lane);
-            acc = acc + load_as<TC>(v) * yj;  // dead for lanes >= j
+            row.add(load_as<TC>(v), yj);  // dead for lanes >= j
         });
-        TC q = quot_fast(xr - acc, dr, rr);
+        TC q = row.q(dr, rr);
         yr = (lane == 0) ? q : yfin;
     }
     return yr;
@@ -364,7 +399,8 @@ __device__ __forceinline__ TC sweep_full_symt(TC xr, TC dr, TC rr, int lane,
     TS ring[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) ring[c] = load_col(c);
-    TC acc = (TC)0, yfin = (TC)0;
+    RowAcc<TC> row(xr);
+    TC yfin = (TC)0;
     static_for<0, NL>([&](auto J) {
         constexpr int j = decltype(J)::value;
         // keep each step's load, LDS store and arithmetic together: hoisted
@@ -373,11 +409,11 @@ __device__ __forceinline__ TC sweep_full_symt(TC xr, TC dr, TC rr, int lane,
         const TS v = ring[j % R];
         if constexpr (j + R < NL) ring[j % R] = load_col(j + R);
         wrow[j] = v;
-        TC q = quot_fast(xr - acc, dr, rr);
+        TC q = row.q(dr, rr);
         TC yj = bcast_step<PM, j>(q, yfin, base4, lane);
-        acc = acc + load_as<TC>(v) * yj;  // dead for lanes <= j
+        row.add(load_as<TC>(v), yj);  // dead for lanes <= j
     });
-    TC q = quot_fast(xr - acc, dr, rr);
+    TC q = row.q(dr, rr);
     TC yr = (lane == PM - 1) ? q : yfin;
     const TC xb = dr * yr;  // y .= D_Dl1 .* y  (:897)
     // the copy is complete; other lanes' rows are read below, RB steps ahead
@@ -387,18 +423,18 @@ __device__ __forceinline__ TC sweep_full_symt(TC xr, TC dr, TC rr, int lane,
     TS rb[RB];
 #pragma unroll
     for (int c = 0; c < RB; ++c) rb[c] = rcol[(PM - 1 - c) * RS];
-    acc = (TC)0;
+    row = RowAcc<TC>(xb);
     yfin = (TC)0;
     static_for<0, NL>([&](auto JJ) {
         constexpr int jj = decltype(JJ)::value;
         constexpr int j = PM - 1 - jj;
         const TS v = rb[jj % RB];
         if constexpr (jj + RB < NL) rb[jj % RB] = rcol[(PM - 1 - (jj + RB)) * RS];
-        TC qb = quot_fast(xb - acc, dr, rr);
+        TC qb = row.q(dr, rr);
         TC yj = bcast_step<PM, j>(qb, yfin, base4, lane);
-        acc = acc + load_as<TC>(v) * yj;  // dead for lanes >= j
+        row.add(load_as<TC>(v), yj);  // dead for lanes >= j
     });
-    q = quot_fast(xb - acc, dr, rr);
+    q = row.q(dr, rr);
     return (lane == 0) ? q : yfin;
 }
 
@@ -411,6 +447,7 @@ __device__ __forceinline__ TC sweep_full_symt(TC xr, TC dr, TC rr, int lane,
 // other row of its line, so at step j lane j's base moves (v_writelane, one
 // operation per step) to G - j, G = 32 * 65: the dead stores of step s then go
 // to G + s - r, distinct for the dead lanes r <= s and inside [G, G + 63).
+// (wbase is kept in bytes.)
 // The backward pass reads row j at a step-constant offset from lds + lane:
 // 65 j (j < 32) or 65 (63 - j) + 64 - j (j >= 32).  Same loads, products and
 // order as sweep_full_symt: bit-identical results.
@@ -424,26 +461,30 @@ __device__ __forceinline__ TC sweep_full_symt64(TC xr, TC dr, TC rr, int lane,
         const int off = (j == 0) ? (lane > 0 ? lane - 1 : 0) : lane - j - 1;
         return Lk[colL_off(j, PM) + off];
     };
-    int wbase = lane < 32 ? lane * RS : (63 - lane) * RS + 64 - lane;
+    // byte offsets, so that a step's store address is wbase plus an immediate
+    constexpr int ES = (int)sizeof(TS);
+    int wbase = (lane < 32 ? lane * RS : (63 - lane) * RS + 64 - lane) * ES;
     asm volatile("" : "+v"(wbase));
+    char *const lbytes = reinterpret_cast<char *>(lds);
     const TS *const rrow = lds + lane;
     const int base4 = seg_base4(PM);
     TS ring[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) ring[c] = load_col(c);
-    TC acc = (TC)0, yfin = (TC)0;
+    RowAcc<TC> row(xr);
+    TC yfin = (TC)0;
     static_for<0, NL>([&](auto J) {
         constexpr int j = decltype(J)::value;
         __builtin_amdgcn_sched_barrier(0);
         const TS v = ring[j % R];
         if constexpr (j + R < NL) ring[j % R] = load_col(j + R);
-        wbase = hs_writelane_i32(G - j, j, wbase);
-        lds[wbase + j] = v;
-        TC q = quot_fast(xr - acc, dr, rr);
+        wbase = hs_writelane_i32((G - j) * ES, j, wbase);
+        *reinterpret_cast<TS *>(lbytes + wbase + j * ES) = v;
+        TC q = row.q(dr, rr);
         TC yj = bcast_step<PM, j>(q, yfin, base4, lane);
-        acc = acc + load_as<TC>(v) * yj;  // dead for lanes <= j
+        row.add(load_as<TC>(v), yj);  // dead for lanes <= j
     });
-    TC q = quot_fast(xr - acc, dr, rr);
+    TC q = row.q(dr, rr);
     TC yr = (lane == PM - 1) ? q : yfin;
     const TC xb = dr * yr;  // y .= D_Dl1 .* y  (:897)
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -452,18 +493,18 @@ __device__ __forceinline__ TC sweep_full_symt64(TC xr, TC dr, TC rr, int lane,
     TS rb[RB];
 #pragma unroll
     for (int c = 0; c < RB; ++c) rb[c] = rrow[row_at(PM - 1 - c)];
-    acc = (TC)0;
+    row = RowAcc<TC>(xb);
     yfin = (TC)0;
     static_for<0, NL>([&](auto JJ) {
         constexpr int jj = decltype(JJ)::value;
         constexpr int j = PM - 1 - jj;
         const TS v = rb[jj % RB];
         if constexpr (jj + RB < NL) rb[jj % RB] = rrow[row_at(PM - 1 - (jj + RB))];
-        TC qb = quot_fast(xb - acc, dr, rr);
+        TC qb = row.q(dr, rr);
         TC yj = bcast_step<PM, j>(qb, yfin, base4, lane);
-        acc = acc + load_as<TC>(v) * yj;  // dead for lanes >= j
+        row.add(load_as<TC>(v), yj);  // dead for lanes >= j
     });
-    q = quot_fast(xb - acc, dr, rr);
+    q = row.q(dr, rr);
     return (lane == 0) ? q : yfin;
 }
 
