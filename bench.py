@@ -182,6 +182,40 @@ def mixed_precision_bench(hs, pb, builder, n3=192, log_fn=log):
                         f"nnz={A.nnz}), seed 2; FP64 CG rtol 1e-8", "solves": rows}
 
 
+def amg_smoother_bench(hs, pb, A, builder, steps, warmup):
+    """AMG.smooth! on the 128^3 problem (2 rounds of x -= M^-1 (A x - b), the
+    extension's L1GS setup: isSymA = false, SymmetricSweep, P = 64): the fused
+    launch per round against the same round composed of separate calls
+    (SpMV, subtraction, sweep, subtraction)."""
+    import torch
+    from hybridsmoothers import amg
+    n = A.shape[0]
+    s = amg.setup_smoother(builder, A, None, partsize=64)
+    b = torch.tensor(pb.uniform_rhs(n, 5), device="cuda")
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    fused = time_kernel(lambda: amg.smooth_(x, s, b, sync=False), steps, warmup, stream)
+    r = torch.empty_like(x)
+    z = torch.empty_like(x)
+
+    def composed():
+        for _ in range(s.iters):
+            s.A.mul(r, x, sync=False)
+            r.sub_(b)
+            hs.ldiv_(z, s.s, r, sync=False)
+            x.sub_(z)
+    unfused = time_kernel(composed, steps, warmup, stream)
+    S = s.s.packed_size
+    nent = int(s.A.nnz)  # entries the rows read (the sliced-ELL padding is not counted)
+    per_round = 2 * S * 8 + nent * 12 + n * (8 + 8 + 8 + 8)  # triangles, A, x, b, D, x_new
+    return {"workload": f"3D 7-point Poisson 128^3, AMG.smooth! with L1GS P=64 (isSymA=false, "
+                        f"SymmetricSweep), {s.iters} rounds per call",
+            "fused_ms": round(fused, 4), "composed_ms": round(unfused, 4),
+            "speedup": round(unfused / fused, 3),
+            "bytes_per_round": per_round,
+            "fused_GBs": round(per_round * s.iters / (fused * 1e-3) / 1e9, 1)}
+
+
 def batch_steady(kind, nsys=256, iters=60):
     """Steady state of the batched engine for a kernel profile: `nsys` equal
     systems (2D 181^2 or 3D 32^3 lognormal, 8.4 M rows in all) resident
@@ -408,6 +442,8 @@ def main():
                 r["speedup_vs_fp64_one_triangle"] = round(r["solves_per_s"] / base1, 3)
         out["pcg_128"] = pcg
         del M
+        log("[bench] AMG smoother on 128^3")
+        out["amg_smoother_128"] = amg_smoother_bench(hs, pb, A, builder, 50, 5)
         if not args.no_mixed:
             out["mixed_precision_192"] = mixed_precision_bench(hs, pb, builder)
     if not args.quick and not args.no_serve:

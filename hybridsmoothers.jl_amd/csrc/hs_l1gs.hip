@@ -339,12 +339,12 @@ __device__ __forceinline__ TC sweep_ring64(TC xr, TC dr, TC rr, int lane,
     return (lane == 0) ? q : yfin;
 }
 
-template <class TS, class TC, int NR, int RUN, bool CG>
+template <class TS, class TC, int NR, int RUN, int FORM>
 __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
                                                      const TS *__restrict__ T,
                                                      const TC *__restrict__ D, int64_t n,
                                                      int64_t nparts, const int *__restrict__ done,
-                                                     CGFuse f) {
+                                                     CGFuse f, SmoothFuse sf = SmoothFuse{}) {
     if (done && *done) return;
     __shared__ TS ring[NR * 64];
     const int lane = threadIdx.x;
@@ -357,7 +357,11 @@ __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
     block_bases(k, 64, Pa, MODE_SYM, lb, ub);
     const bool live = lane < Pa;
     double xin = 0.0;
-    if (live) xin = CG ? cg_prologue(f, s + lane) : x[s + lane];
+    if (live) {
+        if constexpr (FORM == FORM_CG) xin = cg_prologue(f, s + lane);
+        else if constexpr (FORM == FORM_SMOOTH) xin = row_dot(sf.A, sf.x, s + lane) - sf.b[s + lane];
+        else xin = x[s + lane];
+    }
     const TC xr = (TC)xin;
     const TC dr = live ? D[s + lane] : (TC)1;
     const TC rr = (TC)1 / dr;
@@ -369,8 +373,11 @@ __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
     } else {
         yr = sweep_generic<64, MODE_SYM, true, TS, TC>(xr, dr, rr, lane, Pa, live, T + lb, T + ub);
     }
-    if (live) y[s + lane] = (double)yr;
-    if constexpr (CG) {
+    if (live) {
+        if constexpr (FORM == FORM_SMOOTH) sf.xout[s + lane] = sf.x[s + lane] - (double)yr;
+        else y[s + lane] = (double)yr;
+    }
+    if constexpr (FORM == FORM_CG) {
         double v = live ? xin * (double)yr : 0.0;  // r'z partial of this partition
         v = wave_sum(v);
         if (lane == 0) f.partials[k] = v;
@@ -380,41 +387,43 @@ __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
 // One partition per PM-lane segment, 256 threads per workgroup.
 // TS: storage type of the triangles, TC: arithmetic type (double or float).
 // x and y may alias (each partition reads its rows before writing them).
-template <int PM, int MODE, class TS, class TC, bool CG = false>
+template <int PM, int MODE, class TS, class TC, int FORM = FORM_PLAIN>
 __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
                                                    const TS *__restrict__ T,
                                                    const TC *__restrict__ D, int64_t n, int P,
                                                    int64_t nparts, const int *__restrict__ done,
-                                                   CGFuse f = CGFuse{}) {
+                                                   CGFuse f = CGFuse{},
+                                                   SmoothFuse sf = SmoothFuse{}) {
     if (done && *done) return;  // converged CG: speculative launches do nothing
     constexpr int SEGS = 256 / PM;
     const int lane = threadIdx.x % PM;
     const int64_t k = (int64_t)blockIdx.x * SEGS + threadIdx.x / PM;
     const bool wave_live = !__all(k >= nparts);  // the same in every lane of a wave
-    if constexpr (CG) {
+    if constexpr (FORM == FORM_CG) {
         // whole blocks stay for the block-wide r'z partial
         __shared__ double sm[4];
         double v = 0.0;
         if (wave_live)
-            v = sweep_seg_body<PM, MODE, TS, TC, true>(x, y, T, D, n, P, nparts, k, lane, f);
+            v = sweep_seg_body<PM, MODE, TS, TC, FORM>(x, y, T, D, n, P, nparts, k, lane, f, sf);
         v = block_sum(v, sm);
         if (threadIdx.x == 0) f.partials[blockIdx.x] = v;
         return;
     }
     if (!wave_live) return;
-    (void)sweep_seg_body<PM, MODE, TS, TC, false>(x, y, T, D, n, P, nparts, k, lane, f);
+    (void)sweep_seg_body<PM, MODE, TS, TC, FORM>(x, y, T, D, n, P, nparts, k, lane, f, sf);
 }
 
 // One-triangle symmetric sweep (HS_CACHE_PACKED_SYM): one wave per workgroup,
 // 64 / PM partitions per wave, each with its LDS copy of L (symt_lds(PM)
 // elements: 17 KB per wave in FP64 at PM = 64), so that LDS does not limit
 // the waves per CU below what the recurrence's latency needs.
-template <int PM, class TS, class TC, bool CG = false>
+template <int PM, class TS, class TC, int FORM = FORM_PLAIN>
 __global__ __launch_bounds__(64) void k_sweep_symt(const double *x, double *y,
                                                    const TS *__restrict__ T,
                                                    const TC *__restrict__ D, int64_t n, int P,
                                                    int64_t nparts, const int *__restrict__ done,
-                                                   CGFuse f = CGFuse{}) {
+                                                   CGFuse f = CGFuse{},
+                                                   SmoothFuse sf = SmoothFuse{}) {
     if (done && *done) return;
     constexpr int SEGS = 64 / PM;
     __shared__ TS lds[SEGS * symt_lds(PM)];
@@ -422,13 +431,13 @@ __global__ __launch_bounds__(64) void k_sweep_symt(const double *x, double *y,
     const int seg = threadIdx.x / PM;
     const int64_t k = (int64_t)blockIdx.x * SEGS + seg;
     if (__all(k >= nparts)) {
-        if constexpr (CG)
+        if constexpr (FORM == FORM_CG)
             if (threadIdx.x == 0) f.partials[blockIdx.x] = 0.0;
         return;
     }
-    double v = sweep_seg_body<PM, MODE_SYMT, TS, TC, CG>(x, y, T, D, n, P, nparts, k, lane, f,
-                                                         lds + seg * symt_lds(PM));
-    if constexpr (CG) {
+    double v = sweep_seg_body<PM, MODE_SYMT, TS, TC, FORM>(x, y, T, D, n, P, nparts, k, lane, f,
+                                                           sf, lds + seg * symt_lds(PM));
+    if constexpr (FORM == FORM_CG) {
         v = wave_sum(v);  // r'z partial of this workgroup
         if (threadIdx.x == 0) f.partials[blockIdx.x] = v;
     }
@@ -591,44 +600,94 @@ static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *
     }
 }
 
-template <int PM, class TS, class TC, bool CG>
+template <int PM, class TS, class TC, int FORM>
 static unsigned launch_symt_pm(const L1GSPlan &h, const double *x, double *y, const TC *D,
-                               hipStream_t st, const int *done, const CGFuse &f) {
+                               hipStream_t st, const int *done, const CGFuse &f,
+                               const SmoothFuse &sf) {
     constexpr int SEGS = 64 / PM;
     const unsigned grid = grid_for(h.nparts, SEGS);
-    hipLaunchKernelGGL((k_sweep_symt<PM, TS, TC, CG>), dim3(grid), dim3(64), 0, st, x, y,
+    hipLaunchKernelGGL((k_sweep_symt<PM, TS, TC, FORM>), dim3(grid), dim3(64), 0, st, x, y,
                        reinterpret_cast<const TS *>(h.T.get()), D, h.n, (int)h.P, h.nparts, done,
-                       f);
+                       f, sf);
     return grid;
 }
 
 // HS_CACHE_PACKED_SYM (partsize <= 64, checked at build); returns the number
 // of workgroups (= r'z partials in the CG form)
-template <class TS, class TC, bool CG>
+template <class TS, class TC, int FORM>
 static unsigned launch_symt(const L1GSPlan &h, const double *x, double *y, const TC *D,
-                            hipStream_t st, const int *done, const CGFuse &f) {
-    if (h.P <= 4) return launch_symt_pm<4, TS, TC, CG>(h, x, y, D, st, done, f);
-    if (h.P <= 8) return launch_symt_pm<8, TS, TC, CG>(h, x, y, D, st, done, f);
-    if (h.P <= 16) return launch_symt_pm<16, TS, TC, CG>(h, x, y, D, st, done, f);
-    if (h.P <= 32) return launch_symt_pm<32, TS, TC, CG>(h, x, y, D, st, done, f);
-    return launch_symt_pm<64, TS, TC, CG>(h, x, y, D, st, done, f);
+                            hipStream_t st, const int *done, const CGFuse &f,
+                            const SmoothFuse &sf = SmoothFuse{}) {
+    if (h.P <= 4) return launch_symt_pm<4, TS, TC, FORM>(h, x, y, D, st, done, f, sf);
+    if (h.P <= 8) return launch_symt_pm<8, TS, TC, FORM>(h, x, y, D, st, done, f, sf);
+    if (h.P <= 16) return launch_symt_pm<16, TS, TC, FORM>(h, x, y, D, st, done, f, sf);
+    if (h.P <= 32) return launch_symt_pm<32, TS, TC, FORM>(h, x, y, D, st, done, f, sf);
+    return launch_symt_pm<64, TS, TC, FORM>(h, x, y, D, st, done, f, sf);
+}
+
+// The fused smoother round with both triangles or a one-sided sweep.
+template <int PM, class TS, class TC>
+static void launch_seg_smooth(const L1GSPlan &h, const TC *D, hipStream_t st,
+                              const SmoothFuse &sf) {
+    constexpr int SEGS = 256 / PM;
+    const unsigned grid = grid_for(h.nparts, SEGS);
+    const TS *T = reinterpret_cast<const TS *>(h.T.get());
+    const int P = (int)h.P;
+    const double *x = nullptr;
+    double *y = nullptr;
+    const int *done = nullptr;
+    if (h.sweep == HS_SWEEP_FORWARD)
+        hipLaunchKernelGGL((k_sweep_seg<PM, MODE_FWD, TS, TC, FORM_SMOOTH>), dim3(grid), dim3(256),
+                           0, st, x, y, T, D, h.n, P, h.nparts, done, CGFuse{}, sf);
+    else if (h.sweep == HS_SWEEP_BACKWARD)
+        hipLaunchKernelGGL((k_sweep_seg<PM, MODE_BWD, TS, TC, FORM_SMOOTH>), dim3(grid), dim3(256),
+                           0, st, x, y, T, D, h.n, P, h.nparts, done, CGFuse{}, sf);
+    else
+        hipLaunchKernelGGL((k_sweep_seg<PM, MODE_SYM, TS, TC, FORM_SMOOTH>), dim3(grid), dim3(256),
+                           0, st, x, y, T, D, h.n, P, h.nparts, done, CGFuse{}, sf);
 }
 
 template <class TS, class TC>
-static void launch_packed(const L1GSPlan &h, const double *x, double *y, const TC *D,
-                          hipStream_t st, const int *done) {
+static void launch_smooth(const L1GSPlan &h, const TC *D, hipStream_t st, const SmoothFuse &sf) {
     if (h.cache == HS_CACHE_PACKED_SYM) {
-        launch_symt<TS, TC, false>(h, x, y, D, st, done, CGFuse{});
+        launch_symt<TS, TC, FORM_SMOOTH>(h, nullptr, nullptr, D, st, nullptr, CGFuse{}, sf);
         return;
     }
     if (h.P == 64 && h.sweep == HS_SWEEP_SYMMETRIC && h.variant > 0) {
         const TS *T = reinterpret_cast<const TS *>(h.T.get());
         const dim3 grid(grid_for(h.nparts, 1)), block(64);
         if (h.variant == 1)
-            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 8, 12, false>), grid, block, 0, st, x, y, T,
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 8, 12, FORM_SMOOTH>), grid, block, 0, st,
+                               (const double *)nullptr, (double *)nullptr, T, D, h.n, h.nparts,
+                               (const int *)nullptr, CGFuse{}, sf);
+        else
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 16, 24, FORM_SMOOTH>), grid, block, 0, st,
+                               (const double *)nullptr, (double *)nullptr, T, D, h.n, h.nparts,
+                               (const int *)nullptr, CGFuse{}, sf);
+        return;
+    }
+    if (h.P <= 4) launch_seg_smooth<4, TS, TC>(h, D, st, sf);
+    else if (h.P <= 8) launch_seg_smooth<8, TS, TC>(h, D, st, sf);
+    else if (h.P <= 16) launch_seg_smooth<16, TS, TC>(h, D, st, sf);
+    else if (h.P <= 32) launch_seg_smooth<32, TS, TC>(h, D, st, sf);
+    else launch_seg_smooth<64, TS, TC>(h, D, st, sf);
+}
+
+template <class TS, class TC>
+static void launch_packed(const L1GSPlan &h, const double *x, double *y, const TC *D,
+                          hipStream_t st, const int *done) {
+    if (h.cache == HS_CACHE_PACKED_SYM) {
+        launch_symt<TS, TC, FORM_PLAIN>(h, x, y, D, st, done, CGFuse{});
+        return;
+    }
+    if (h.P == 64 && h.sweep == HS_SWEEP_SYMMETRIC && h.variant > 0) {
+        const TS *T = reinterpret_cast<const TS *>(h.T.get());
+        const dim3 grid(grid_for(h.nparts, 1)), block(64);
+        if (h.variant == 1)
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 8, 12, FORM_PLAIN>), grid, block, 0, st, x, y, T,
                                D, h.n, h.nparts, done, CGFuse{});
         else
-            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 16, 24, false>), grid, block, 0, st, x, y, T,
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 16, 24, FORM_PLAIN>), grid, block, 0, st, x, y, T,
                                D, h.n, h.nparts, done, CGFuse{});
         return;
     }
@@ -669,15 +728,15 @@ static unsigned launch_fused(const L1GSPlan &h, double *z, const TC *D, hipStrea
     const int *done = reinterpret_cast<const int *>(reinterpret_cast<const char *>(f.st) +
                                                     offsetof(CGDev, done));
     if (h.cache == HS_CACHE_PACKED_SYM)
-        return launch_symt<TS, TC, true>(h, (const double *)nullptr, z, D, st, done, f);
+        return launch_symt<TS, TC, FORM_CG>(h, (const double *)nullptr, z, D, st, done, f);
     if (h.P == 64 && h.variant > 0) {
         const TS *T = reinterpret_cast<const TS *>(h.T.get());
         const dim3 grid(grid_for(h.nparts, 1)), block(64);
         if (h.variant == 1)
-            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 8, 12, true>), grid, block, 0, st,
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 8, 12, FORM_CG>), grid, block, 0, st,
                                (const double *)nullptr, z, T, D, h.n, h.nparts, done, f);
         else
-            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 16, 24, true>), grid, block, 0, st,
+            hipLaunchKernelGGL((k_sweep_ring64<TS, TC, 16, 24, FORM_CG>), grid, block, 0, st,
                                (const double *)nullptr, z, T, D, h.n, h.nparts, done, f);
         return (unsigned)h.nparts;
     }
@@ -691,6 +750,17 @@ static unsigned launch_fused(const L1GSPlan &h, double *z, const TC *D, hipStrea
 bool l1gs_can_fuse(const L1GSPlan &h) {
     return (h.cache == HS_CACHE_PACKED || h.cache == HS_CACHE_PACKED_SYM) &&
            h.sweep == HS_SWEEP_SYMMETRIC && h.P <= 64 && h.n > 0;
+}
+
+bool l1gs_can_smooth_fused(const L1GSPlan &h) {
+    return h.cache != HS_CACHE_VIEW && h.P <= 64 && h.n > 0;
+}
+
+void l1gs_smooth_fused(const L1GSPlan &h, const SmoothFuse &sf, hipStream_t st) {
+    if (h.prec == HS_PREC_FP64) launch_smooth<double, double>(h, h.D.get(), st, sf);
+    else if (h.prec == HS_PREC_FP32) launch_smooth<float, float>(h, h.Df.get(), st, sf);
+    else launch_smooth<__hip_bfloat16, float>(h, h.Df.get(), st, sf);
+    check_launch();
 }
 
 unsigned l1gs_apply_fused(const L1GSPlan &h, double *z, hipStream_t st, const CGFuse &f) {

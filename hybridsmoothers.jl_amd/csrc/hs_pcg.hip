@@ -402,6 +402,20 @@ void l1gs_smooth_device(const CSR &A, const L1GSPlan &P, double *x, const double
                         double *r, double *z, hipStream_t st) {
     const int64_t n = A.n;
     const unsigned g = grid_for(n, 256);
+    if (l1gs_can_smooth_fused(P)) {
+        // one launch per round: residual, sweep and update fused; the rounds
+        // alternate between x and r, and an odd count ends with a copy to x
+        double *cur = x, *next = r;
+        for (int it = 0; it < iters; ++it) {
+            l1gs_smooth_fused(P, SmoothFuse{A.view(), cur, b, next}, st);
+            std::swap(cur, next);
+        }
+        if (cur != x) {
+            hipLaunchKernelGGL(k_copy, dim3(g), dim3(256), 0, st, n, cur, x, (const CGDev *)nullptr);
+            check_launch();
+        }
+        return;
+    }
     for (int it = 0; it < iters; ++it) {
         hipLaunchKernelGGL(k_resid_minus, dim3(g), dim3(256), 0, st, A.view(), x, b, r);
         check_launch();

@@ -126,6 +126,18 @@ struct CGFuse {
     const CGDev *st;
 };
 
+// The fused smoother form, one round of AMG.smooth! (ext/HybridSmoothersAMGExt.jl:
+// 29-35): r_i = (A x)_i - b_i computed in the sweep's prologue (row sum in
+// column order, then the subtraction, as mul! followed by .-= rounds them),
+// z = M^-1 r, and x_new_i = x_i - z_i stored to xout (not x: other partitions
+// still read x for their residuals).
+struct SmoothFuse {
+    SellView<double> A;
+    const double *x;
+    const double *b;
+    double *xout;
+};
+
 struct L1GSPlan;
 struct ChebPlan;
 
@@ -148,6 +160,11 @@ int64_t cheb_rows(const ChebPlan &s);
 uint64_t cheb_serial(const ChebPlan &s);
 
 void spmv_device(const CSR &A, const double *x, double *y, hipStream_t st);
+
+// One fused round xout = x - M^-1 (A x - b) (packed layouts, partsize <= 64:
+// check l1gs_can_smooth_fused first); x and xout must not alias.
+bool l1gs_can_smooth_fused(const L1GSPlan &h);
+void l1gs_smooth_fused(const L1GSPlan &h, const SmoothFuse &sf, hipStream_t st);
 
 // `iters` times x -= M^-1 (A x - b) on device vectors (r, z: n-vectors of scratch).
 void l1gs_smooth_device(const CSR &A, const L1GSPlan &P, double *x, const double *b, int iters,

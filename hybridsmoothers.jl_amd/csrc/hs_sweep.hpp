@@ -534,11 +534,13 @@ __device__ __forceinline__ double cg_prologue(const CGFuse &f, int64_t i) {
 // are identity rows (d = 1, x = 0): they stay exactly 0 and add exact zero
 // terms to the real rows, whose results are therefore those of the short
 // partition's own recurrence.
-template <int PM, int MODE, class TS, class TC, bool PAD, class In>
-__device__ __forceinline__ double sweep_seg_run(In &&input, double *y, const TS *__restrict__ T,
-                                                const TC *__restrict__ D, int64_t n, int P,
-                                                int64_t nparts, int64_t k, int lane,
-                                                TS *lds = nullptr) {
+// `output(i, y_i)` stores the lane's result (sweep_seg_run: y[i] = y_i).
+template <int PM, int MODE, class TS, class TC, bool PAD, class In, class Out>
+__device__ __forceinline__ double sweep_seg_run_io(In &&input, Out &&output,
+                                                   const TS *__restrict__ T,
+                                                   const TC *__restrict__ D, int64_t n, int P,
+                                                   int64_t nparts, int64_t k, int lane,
+                                                   TS *lds = nullptr) {
     const bool valid = k < nparts;
     const int64_t kk = valid ? k : nparts - 1;
     const int64_t s = kk * (int64_t)P;
@@ -569,21 +571,39 @@ __device__ __forceinline__ double sweep_seg_run(In &&input, double *y, const TS 
     } else {
         yr = sweep_generic<PM, MODE, true, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
     }
-    if (live) y[s + lane] = (double)yr;
+    if (live) output(s + lane, (double)yr);
     return live ? xin * (double)yr : 0.0;
 }
 
-// The plain form (input x) and the fused CG form (input r - alpha q, with the
-// CG vector updates of cg_prologue).
-template <int PM, int MODE, class TS, class TC, bool CG>
+template <int PM, int MODE, class TS, class TC, bool PAD, class In>
+__device__ __forceinline__ double sweep_seg_run(In &&input, double *y, const TS *__restrict__ T,
+                                                const TC *__restrict__ D, int64_t n, int P,
+                                                int64_t nparts, int64_t k, int lane,
+                                                TS *lds = nullptr) {
+    return sweep_seg_run_io<PM, MODE, TS, TC, PAD>(
+        input, [&](int64_t i, double v) { y[i] = v; }, T, D, n, P, nparts, k, lane, lds);
+}
+
+// Forms of the single-system sweep kernels.
+enum { FORM_PLAIN = 0, FORM_CG = 1, FORM_SMOOTH = 2 };
+
+// The plain form (input x), the fused CG form (input r - alpha q, with the
+// CG vector updates of cg_prologue) and the fused smoother form.
+template <int PM, int MODE, class TS, class TC, int FORM>
 __device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
                                                  const TS *__restrict__ T,
                                                  const TC *__restrict__ D, int64_t n, int P,
                                                  int64_t nparts, int64_t k, int lane,
-                                                 const CGFuse &f, TS *lds = nullptr) {
-    if constexpr (CG)
+                                                 const CGFuse &f, const SmoothFuse &sf,
+                                                 TS *lds = nullptr) {
+    if constexpr (FORM == FORM_CG)
         return sweep_seg_run<PM, MODE, TS, TC, false>([&](int64_t i) { return cg_prologue(f, i); },
                                                       y, T, D, n, P, nparts, k, lane, lds);
+    else if constexpr (FORM == FORM_SMOOTH)
+        return sweep_seg_run_io<PM, MODE, TS, TC, false>(
+            [&](int64_t i) { return row_dot(sf.A, sf.x, i) - sf.b[i]; },
+            [&](int64_t i, double z) { sf.xout[i] = sf.x[i] - z; }, T, D, n, P, nparts, k, lane,
+            lds);
     else
         return sweep_seg_run<PM, MODE, TS, TC, false>([&](int64_t i) { return x[i]; }, y, T, D, n,
                                                       P, nparts, k, lane, lds);

@@ -43,8 +43,10 @@ def setup_smoother(config, A, symmetry=None, partsize=None):
     return HybridSmoother(DeviceMatrix(A), P)
 
 
-def smooth_(x, s, b, stream=None):
-    """AMG.smooth!(x, s, b): x updated in place (numpy = host, torch CUDA = device)."""
+def smooth_(x, s, b, stream=None, sync=True):
+    """AMG.smooth!(x, s, b): x updated in place (numpy = host, torch CUDA = device).
+    With packed triangles and partsize <= 64 each round is one fused launch
+    (residual, sweep and update); sync=False leaves device work queued."""
     if isinstance(s, ChebyshevSmoother):
         return _cheb_smooth(x, s, b)
     n = s.A.n
@@ -53,4 +55,5 @@ def smooth_(x, s, b, stream=None):
     if lx != lb:
         raise L.HSError(L.HS_EINVAL, "x and b must both be host or both be device vectors")
     st = stream if stream is not None else (L.current_stream() if lx == L.LOC_DEVICE else None)
-    L.check(L.lib.hs_l1gs_smooth(s.s.handle, s.A.handle, px, pb, s.iters, lx, st, 0))
+    L.check(L.lib.hs_l1gs_smooth(s.s.handle, s.A.handle, px, pb, s.iters, lx, st,
+                                 0 if sync else L.FLAG_ASYNC))

@@ -37,12 +37,14 @@ class DeviceMatrix:
     def handle(self):
         return self._h
 
-    def mul(self, y, x, stream=None):
+    def mul(self, y, x, stream=None, sync=True):
         py, ly = L.vec_ptr(y, self.n)
         px, lx = L.vec_ptr(x, self.n)
         if ly != lx:
             raise L.HSError(L.HS_EINVAL, "x and y must both be host or both be device vectors")
-        L.check(L.lib.hs_csr_spmv(self._h, py, px, ly, stream, 0))
+        if stream is None and ly == L.LOC_DEVICE:
+            stream = L.current_stream()
+        L.check(L.lib.hs_csr_spmv(self._h, py, px, ly, stream, 0 if sync else L.FLAG_ASYNC))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -92,3 +92,38 @@ def test_chebyshev_amg_smoother(gpu):
     amg.smooth_(x, s, b)
     hs.smooth_(y, s, b)
     np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("sweep", ["forward", "backward", "symmetric"])
+@pytest.mark.parametrize("partsize,cache,iters", [(3, "packed", 1), (32, "packed", 2),
+                                                  (64, "packed", 3), (64, "view", 2),
+                                                  (100, "packed", 2), (16, "sym1", 2)])
+def test_fused_smoother_rounds(gpu, sweep, partsize, cache, iters):
+    """Each round of hs_l1gs_smooth is one fused launch for packed layouts with
+    partsize <= 64 (residual in the sweep prologue, x - z stored to the other
+    of two buffers, a final copy after an odd count); the view cache and larger
+    partsizes take the three-launch path.  Every case equals the composition of
+    separate calls bit for bit."""
+    import torch
+    from hybridsmoothers import amg, problems as pb
+    hs = gpu
+    if cache == "sym1" and sweep != "symmetric":
+        pytest.skip("the one-triangle cache is a symmetric-sweep layout")
+    A = pb.lognormal3d(14, seed=12)
+    n = A.shape[0]
+    sw = {"forward": hs.ForwardSweep(), "backward": hs.BackwardSweep(),
+          "symmetric": hs.SymmetricSweep()}[sweep]
+    strategy = {"packed": hs.PackedBufferCache(), "view": hs.MatrixViewCache(),
+                "sym1": hs.PackedSymmetricCache()}[cache]
+    P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=16)(
+        A, partsize, isSymA=True, sweep=sw, cache_strategy=strategy)
+    s = amg.HybridSmoother(hs.DeviceMatrix(A), P, iters=iters)
+    b = pb.uniform_rhs(n, 21)
+    x0 = pb.uniform_rhs(n, 22)
+    x = x0.copy()
+    amg.smooth_(x, s, b)
+    np.testing.assert_array_equal(x, _compose(s.A, P, x0, b, hs, iters=iters))
+    xd = torch.tensor(x0, device="cuda")
+    amg.smooth_(xd, s, torch.tensor(b, device="cuda"), sync=False)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(xd.cpu().numpy(), x)
