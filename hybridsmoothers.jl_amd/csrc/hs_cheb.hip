@@ -357,12 +357,7 @@ int hs_cheb_build(int64_t n, const int64_t *colptr, const int64_t *rowval, const
         s->degree = degree;
         s->prec = precision;
         s->n = n;
-        {
-            std::vector<int64_t> rp, ci;
-            std::vector<double> cv;
-            csc_to_csr_host(n, colptr, rowval, nzval, rp, ci, cv);  // SparseMatrixCSR(A) (:144)
-            csr_upload(s->A, n, rp.data(), ci.data(), cv.data(), st);
-        }
+        csr_create_device(s->A, n, colptr, rowval, nzval, HS_FMT_CSC, st);  // SparseMatrixCSR(A) (:144)
         s->l1inv.alloc(n);
         if (precision == HS_PREC_FP32) {
             s->l1inv32.alloc(n);

@@ -803,48 +803,29 @@ void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_
 
 // D_Dl1 of a non-symmetric CSC: the reference scans every column for row i
 // (:1135-1216), visiting row i's entries in ascending column order.  The same
-// sums come from the rows of the CSR transpose (a stable counting sort keeps
-// ascending columns inside each row), in O(nnz) instead of O(N^2).
-static void nonsym_D(L1GSPlan &h, const int64_t *h_ptr, const int64_t *h_ind, const double *h_val,
-                     double eta, hipStream_t st) {
+// sums come from the rows of the CSR transpose (a stable sort by row keeps
+// ascending columns inside each row), in O(nnz) instead of O(N^2), on the
+// device.
+static void nonsym_D(L1GSPlan &h, int64_t nnz, const DBuf<int64_t> &ptr, const DBuf<int64_t> &ind,
+                     const DBuf<double> &val, double eta, hipStream_t st) {
     const int64_t n = h.n;
-    const int64_t nnz = h_ptr[n] - 1;
-    std::vector<int64_t> rp(n + 1, 0), ci(nnz > 0 ? nnz : 1);
-    std::vector<double> cv(nnz > 0 ? nnz : 1);
-    for (int64_t p = 0; p < nnz; ++p) rp[h_ind[p]] += 1;  // 1-based row r counted in rp[r]
-    rp[0] = 1;
-    for (int64_t i = 1; i <= n; ++i) rp[i] += rp[i - 1];
-    {
-        std::vector<int64_t> cur(rp.begin(), rp.end() - 1);
-        for (int64_t col = 1; col <= n; ++col)
-            for (int64_t p = h_ptr[col - 1]; p <= h_ptr[col] - 1; ++p) {
-                int64_t r = h_ind[p - 1];
-                int64_t q = cur[r - 1]++;
-                ci[q - 1] = col;
-                cv[q - 1] = h_val[p - 1];
-            }
-    }
-    DBuf<int64_t> trp(n + 1), tci(nnz);
-    DBuf<double> tcv(nnz);
-    HS_CHECK_HIP(hipMemcpyAsync(trp.get(), rp.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
-    if (nnz) {
-        HS_CHECK_HIP(hipMemcpyAsync(tci.get(), ci.data(), nnz * 8, hipMemcpyHostToDevice, st));
-        HS_CHECK_HIP(hipMemcpyAsync(tcv.get(), cv.data(), nnz * 8, hipMemcpyHostToDevice, st));
-    }
+    DBuf<int64_t> trp, tci;
+    DBuf<double> tcv;
+    csc_transpose_device(n, nnz, ptr.get(), ind.get(), val.get(), trp, tci, tcv, st);
     hipLaunchKernelGGL((k_setup_rows<double>), dim3(grid_for(n, 256)), dim3(256), 0, st, n, h.P,
                        trp.get(), tci.get(), tcv.get(), eta, h.D.get(), (double *)nullptr,
                        h.sweep, 0, 0);
     check_launch();
-    HS_CHECK_HIP(hipStreamSynchronize(st));  // the host arrays above go out of scope
+    HS_CHECK_HIP(hipStreamSynchronize(st));  // the transpose goes out of scope
 }
 
 int64_t l1gs_rows(const L1GSPlan &h) { return h.n; }
 uint64_t l1gs_serial(const L1GSPlan &h) { return h.serial; }
 
 template <class TS>
-static void build_packed(L1GSPlan &h, const DBuf<int64_t> &ptr, const DBuf<int64_t> &ind,
-                         const DBuf<double> &val, const int64_t *h_ptr, const int64_t *h_ind,
-                         const double *h_val, double eta, hipStream_t st) {
+static void build_packed(L1GSPlan &h, int64_t nnz, const DBuf<int64_t> &ptr,
+                         const DBuf<int64_t> &ind, const DBuf<double> &val, double eta,
+                         hipStream_t st) {
     const int want_L = h.sweep != HS_SWEEP_BACKWARD;
     const int want_U = h.sweep != HS_SWEEP_FORWARD && h.cache != HS_CACHE_PACKED_SYM;
     const int mode = h.tri_mode();
@@ -861,7 +842,7 @@ static void build_packed(L1GSPlan &h, const DBuf<int64_t> &ptr, const DBuf<int64
         check_launch();
         return;
     }
-    nonsym_D(h, h_ptr, h_ind, h_val, eta, st);
+    nonsym_D(h, nnz, ptr, ind, val, eta, st);
     hipLaunchKernelGGL((k_pack_cols<TS>), dim3(grid), dim3(256), 0, st, h.n, h.P, ptr.get(),
                        ind.get(), val.get(), T, mode, want_L, want_U);
     check_launch();
@@ -929,6 +910,8 @@ int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const doubl
                 HS_CHECK_HIP(hipMemcpyAsync(dind.get(), ind, nnz * 8, hipMemcpyHostToDevice, st));
                 HS_CHECK_HIP(hipMemcpyAsync(dval.get(), val, nnz * 8, hipMemcpyHostToDevice, st));
             }
+            // every kernel below indexes through these arrays
+            validate_sparse_device(n, nnz, dptr.get(), dind.get(), st);
         }
         DBuf<int64_t> ddiag;
         if (!h->rowwise && n > 0) {
@@ -952,11 +935,11 @@ int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const doubl
         if (n > 0) {
             if (cache != HS_CACHE_VIEW) {
                 if (precision == HS_PREC_FP64)
-                    build_packed<double>(*h, dptr, dind, dval, ptr, ind, val, eta, st);
+                    build_packed<double>(*h, nnz, dptr, dind, dval, eta, st);
                 else if (precision == HS_PREC_FP32)
-                    build_packed<float>(*h, dptr, dind, dval, ptr, ind, val, eta, st);
+                    build_packed<float>(*h, nnz, dptr, dind, dval, eta, st);
                 else
-                    build_packed<__hip_bfloat16>(*h, dptr, dind, dval, ptr, ind, val, eta, st);
+                    build_packed<__hip_bfloat16>(*h, nnz, dptr, dind, dval, eta, st);
             } else {
                 // MatrixViewCache: D_Dl1 only; keep the matrix on the device
                 if (h->rowwise) {
@@ -965,7 +948,7 @@ int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const doubl
                                        h->D.get(), (double *)nullptr, h->sweep, 0, 0);
                     check_launch();
                 } else {
-                    nonsym_D(*h, ptr, ind, val, eta, st);
+                    nonsym_D(*h, nnz, dptr, dind, dval, eta, st);
                 }
                 // the view kernels read 0-based diagonal positions
                 h->vptr = std::move(dptr);

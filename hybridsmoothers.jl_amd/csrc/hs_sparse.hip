@@ -7,80 +7,10 @@
 
 namespace hs {
 
-void csc_to_csr_host(int64_t n, const int64_t *colptr, const int64_t *rowval, const double *nzval,
-                     std::vector<int64_t> &rp, std::vector<int64_t> &ci, std::vector<double> &cv) {
-    const int64_t nnz = colptr[n] - 1;
-    rp.assign(n + 1, 0);
-    ci.resize(nnz);
-    cv.resize(nnz);
-    for (int64_t p = 0; p < nnz; ++p) rp[rowval[p]] += 1;
-    rp[0] = 1;
-    for (int64_t i = 1; i <= n; ++i) rp[i] += rp[i - 1];
-    std::vector<int64_t> cur(rp.begin(), rp.end() - 1);
-    for (int64_t c = 1; c <= n; ++c)
-        for (int64_t p = colptr[c - 1]; p <= colptr[c] - 1; ++p) {
-            int64_t q = cur[rowval[p - 1] - 1]++;
-            ci[q - 1] = c;
-            cv[q - 1] = nzval[p - 1];
-        }
-}
-
 template <class TD>
 __global__ void k_to_type(int64_t n, const double *__restrict__ s, TD *__restrict__ d) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = (TD)s[i];
-}
-
-void csr_upload(CSR &A, int64_t n, const int64_t *rp1, const int64_t *ci1, const double *cv,
-                hipStream_t st) {
-    HS_REQUIRE(n < (int64_t)INT32_MAX, HS_EUNSUPPORTED, "more than 2^31-1 rows");
-    const int64_t nnz = rp1[n] - 1;
-    A.n = n;
-    A.nnz = nnz;
-    A.nslices = (n + 63) / 64;
-    A.h_rp.resize(n + 1);
-    A.h_ci.resize(nnz);
-    A.h_cv.assign(cv, cv + nnz);
-    for (int64_t i = 0; i <= n; ++i) A.h_rp[i] = rp1[i] - 1;
-    for (int64_t p = 0; p < nnz; ++p) A.h_ci[p] = (int32_t)(ci1[p] - 1);
-    // slice widths and offsets
-    std::vector<int64_t> sp(A.nslices + 1, 0);
-    for (int64_t s = 0; s < A.nslices; ++s) {
-        int64_t w = 0;
-        for (int64_t i = 64 * s; i < std::min<int64_t>(64 * s + 64, n); ++i)
-            w = std::max<int64_t>(w, A.h_rp[i + 1] - A.h_rp[i]);
-        sp[s + 1] = sp[s] + 64 * w;
-    }
-    A.nent = sp[A.nslices];
-    std::vector<int32_t> col(A.nent);
-    std::vector<double> val(A.nent);
-    for (int64_t s = 0; s < A.nslices; ++s) {
-        const int64_t w = (sp[s + 1] - sp[s]) / 64;
-        for (int l = 0; l < 64; ++l) {
-            const int64_t i = 64 * s + l;
-            const int64_t len = i < n ? A.h_rp[i + 1] - A.h_rp[i] : 0;
-            for (int64_t k = 0; k < w; ++k) {
-                const int64_t q = sp[s] + 64 * k + l;
-                if (k < len) {
-                    col[q] = A.h_ci[A.h_rp[i] + k];
-                    val[q] = A.h_cv[A.h_rp[i] + k];
-                } else {  // padding: the row's own column, value zero
-                    col[q] = (int32_t)(i < n ? i : n - 1);
-                    val[q] = 0.0;
-                }
-            }
-        }
-    }
-    A.sptr.alloc(A.nslices + 1);
-    A.col.alloc(A.nent);
-    A.val.alloc(A.nent);
-    HS_CHECK_HIP(hipMemcpyAsync(A.sptr.get(), sp.data(), (A.nslices + 1) * 8,
-                                hipMemcpyHostToDevice, st));
-    if (A.nent) {
-        HS_CHECK_HIP(hipMemcpyAsync(A.col.get(), col.data(), A.nent * 4, hipMemcpyHostToDevice, st));
-        HS_CHECK_HIP(hipMemcpyAsync(A.val.get(), val.data(), A.nent * 8, hipMemcpyHostToDevice, st));
-    }
-    HS_CHECK_HIP(hipStreamSynchronize(st));  // the host staging vectors go out of scope
 }
 
 void CSR::ensure_fp32(hipStream_t st) {
@@ -119,16 +49,7 @@ int hs_csr_create(int64_t n, const int64_t *ptr, const int64_t *ind, const doubl
         HS_REQUIRE(n == 0 || (ptr && ind && val), HS_EINVAL, "matrix arrays are NULL");
         hipStream_t st = as_stream(stream);
         std::unique_ptr<hs_csr> A(new hs_csr());
-        if (n > 0) {
-            if (fmt == HS_FMT_CSR) {
-                csr_upload(*A, n, ptr, ind, val, st);
-            } else {
-                std::vector<int64_t> rp, ci;
-                std::vector<double> cv;
-                csc_to_csr_host(n, ptr, ind, val, rp, ci, cv);
-                csr_upload(*A, n, rp.data(), ci.data(), cv.data(), st);
-            }
-        }
+        csr_create_device(*A, n, ptr, ind, val, fmt, st);
         *out = A.release();
     });
 }
@@ -164,15 +85,7 @@ int hs_csr_info(hs_csr_t A, int64_t *out) {
 int hs_csr_get(hs_csr_t A, int64_t *rowptr, int64_t *colval, double *nzval) {
     return guard([&] {
         HS_REQUIRE(A && rowptr && (A->nnz == 0 || (colval && nzval)), HS_EINVAL, "NULL argument");
-        if (A->n == 0) {
-            rowptr[0] = 1;
-            return;
-        }
-        for (int64_t i = 0; i <= A->n; ++i) rowptr[i] = A->h_rp[i] + 1;
-        for (int64_t p = 0; p < A->nnz; ++p) {
-            colval[p] = (int64_t)A->h_ci[p] + 1;
-            nzval[p] = A->h_cv[p];
-        }
+        csr_download(*A, rowptr, colval, nzval);
     });
 }
 

@@ -15,7 +15,8 @@ namespace hs {
 // (column = the row itself, value = 0).  One wave reads one slice with fully
 // coalesced loads, and every row is still summed in its CSR order (padding
 // comes last and adds zero), so results equal a row-by-row CSR product.
-// A host copy of the CSR (0-based) is kept for getters.
+// The operator is built on the device (hs_setup.hip); the row lengths are
+// kept so that the CSR can be read back (csr_download).
 // ---------------------------------------------------------------------------
 struct CGWork;                 // CG workspace (hs_pcg.hip), owned by the matrix
 void cgwork_free(CGWork *w);
@@ -35,9 +36,7 @@ struct CSR {
     DBuf<int32_t> col;
     DBuf<double> val;
     DBuf<float> val32;
-    std::vector<int64_t> h_rp;  // host CSR, 0-based
-    std::vector<int32_t> h_ci;
-    std::vector<double> h_cv;
+    DBuf<int32_t> rowlen;
     CGWork *cg = nullptr;
 
     CSR() = default;
@@ -57,14 +56,26 @@ struct CSR {
     int64_t bytes(int vbytes) const { return nent * (vbytes + 4) + (nslices + 1) * 8; }
 };
 
-// Host CSR (1-based Int64, as SparseMatrixCSR stores it) from 1-based CSC by a
-// stable counting sort on rows: columns ascend inside every row.
-void csc_to_csr_host(int64_t n, const int64_t *colptr, const int64_t *rowval, const double *nzval,
-                     std::vector<int64_t> &rp, std::vector<int64_t> &ci, std::vector<double> &cv);
-
-// Builds the device operator from a 1-based CSR (host).
-void csr_upload(CSR &A, int64_t n, const int64_t *rp1, const int64_t *ci1, const double *cv,
-                hipStream_t st);
+// Device setup (hs_setup.hip).  Arrays are 1-based Int64 as SparseMatrixCSC /
+// SparseMatrixCSR hold them.
+// HS_EINVAL unless ptr1 starts at 1 and never decreases (nnz = ptr1[n] - 1)
+// and every index is in [1, n] (device arrays; checked before any other
+// kernel reads them).
+void validate_sparse_device(int64_t n, int64_t nnz, const int64_t *ptr1, const int64_t *ind1,
+                            hipStream_t st);
+// CSR of a device CSC (stable by rows: columns ascend inside every row).
+void csc_transpose_device(int64_t n, int64_t nnz, const int64_t *colptr1, const int64_t *rowval1,
+                          const double *nzval, DBuf<int64_t> &rp1, DBuf<int64_t> &ci1,
+                          DBuf<double> &cv, hipStream_t st);
+// The sliced-ELL operator of a device CSR.
+void sell_build_device(CSR &A, int64_t n, int64_t nnz, const int64_t *rp1, const int64_t *ci1,
+                       const double *cv, hipStream_t st);
+// The operator of host CSC (fmt HS_FMT_CSC) or CSR arrays: upload, validate,
+// transpose if needed, pack.
+void csr_create_device(CSR &A, int64_t n, const int64_t *ptr, const int64_t *ind,
+                       const double *val, int fmt, hipStream_t st);
+// The operator's CSR back on the host (1-based; rowptr1 has n + 1 entries).
+void csr_download(const CSR &A, int64_t *rowptr1, int64_t *colval1, double *nzval);
 
 // ---------------------------------------------------------------------------
 // row products, summed in row order
