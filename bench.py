@@ -363,6 +363,13 @@ def main():
             variants.append({"name": f"128^3 P={P} symmetric {pname}", "ms": round(m, 5),
                              "bytes": b, "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
             del pv
+        # partsize 128: one wave per partition, two rows per lane
+        pw = builder(A, 128, isSymA=True, sweep=hs.SymmetricSweep())
+        m = time_kernel(lambda: hs.ldiv_(y, pw, x, sync=False), args.steps, args.warmup, stream)
+        b = sweep_bytes(n, pw.packed_size)
+        variants.append({"name": "128^3 P=128 symmetric fp64", "ms": round(m, 5), "bytes": b,
+                         "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
+        del pw
         # one stored triangle (U = L^T): same results, half the packed bytes
         for pname, tb, db in (("fp64", 8, 8), ("fp32", 4, 4), ("bf16", 2, 4)):
             pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname,

@@ -443,6 +443,121 @@ __global__ __launch_bounds__(64) void k_sweep_symt(const double *x, double *y,
     }
 }
 
+// Partitions of 65 to 64 R rows: one wave per partition, lane l owning rows
+// l, l + 64, ..., l + 64 (R - 1).  Step j of the right-looking recurrence:
+// the owner of row j (lane j mod 64, slot j / 64) forms its quotient, it is
+// broadcast from that lane, and every lane adds L(i, j) y_j to its rows
+// i > j -- one coalesced load per slot from the packed column j.  Each row
+// therefore sums its terms in ascending j (descending in the backward pass),
+// the order of the 64-row kernels and of the reference; with FAST = false
+// (a wave holding an unsafe diagonal) quotients are true divisions.
+template <int R, int MODE, bool FAST, class TS, class TC>
+__device__ __forceinline__ void sweep_wide(TC (&xr)[R], const TC (&dr)[R], const TC (&rr)[R],
+                                           int lane, int Pa, const TS *__restrict__ Lk,
+                                           const TS *__restrict__ Uk) {
+    // Steps run in groups of U; the column values of the next group are
+    // loaded while the current group computes (U R values per lane in
+    // flight).  Lanes without a term at a step (and steps past the last
+    // column) load element 0 of the block.
+    constexpr int U = 16 / R > 2 ? 16 / R : 2;
+    auto pass = [&](auto Upper) {
+        constexpr bool UP = decltype(Upper)::value;
+        const TS *__restrict__ B = UP ? Uk : Lk;
+        auto step_of = [&](int t) { return UP ? Pa - 1 - t : t; };  // t-th step's column
+        auto load_group = [&](int t0, TS (&buf)[U][R]) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = step_of(t0 + u);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int i = lane + 64 * r;
+                    const bool has = (t0 + u < Pa) && (UP ? i < j : (i > j && i < Pa));
+                    const int64_t off = UP ? colU_off(j, Pa) + i : colL_off(j, Pa) + (i - j - 1);
+                    buf[u][r] = B[has ? off : 0];
+                }
+            }
+        };
+        RowAcc<TC> row[R];
+        TC yfin[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            row[r] = RowAcc<TC>(xr[r]);
+            yfin[r] = (TC)0;
+        }
+        TS cur[U][R], nxt[U][R];
+        load_group(0, cur);
+        for (int t0 = 0; t0 < Pa; t0 += U) {
+            load_group(t0 + U, nxt);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u;
+                if (t < Pa) {
+                    const int j = step_of(t);
+                    const int js = j >> 6, jl = j & 63;
+                    TC q = (TC)0;
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (r == js) q = row[r].template q<FAST>(dr[r], rr[r]);
+                    const TC yj = seg_bcast<64>(q, jl);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        if (r == js && lane == jl) yfin[r] = q;
+                        const int i = lane + 64 * r;
+                        if (UP ? i < j : (i > j && i < Pa)) row[r].add(load_as<TC>(cur[u][r]), yj);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int r = 0; r < R; ++r) cur[u][r] = nxt[u][r];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) xr[r] = yfin[r];
+    };
+    if constexpr (MODE != MODE_BWD) pass(std::false_type{});
+    if constexpr (MODE == MODE_SYM) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) xr[r] = dr[r] * xr[r];  // y .= D_Dl1 .* y (:897)
+    }
+    if constexpr (MODE != MODE_FWD) pass(std::true_type{});
+}
+
+template <int R, int MODE, class TS, class TC>
+__global__ __launch_bounds__(64) void k_sweep_wide(const double *x, double *y,
+                                                   const TS *__restrict__ T,
+                                                   const TC *__restrict__ D, int64_t n, int64_t P,
+                                                   int64_t nparts, const int *__restrict__ done) {
+    if (done && *done) return;
+    const int64_t k = blockIdx.x;
+    if (k >= nparts) return;
+    const int lane = threadIdx.x;
+    const int64_t s = k * P;
+    const int Pa = (int)(n - s < P ? n - s : P);
+    int64_t lb, ub;
+    block_bases(k, P, Pa, MODE, lb, ub);
+    TC xr[R], dr[R], rr[R];
+    bool unsafe = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        const bool live = i < Pa;
+        xr[r] = live ? (TC)x[s + i] : (TC)0;
+        dr[r] = live ? D[s + i] : (TC)1;
+        rr[r] = (TC)1 / dr[r];
+        unsafe = unsafe || !safe_divisor(dr[r]);
+    }
+    if (__any(unsafe))
+        sweep_wide<R, MODE, false, TS, TC>(xr, dr, rr, lane, Pa, T + lb, T + ub);
+    else
+        sweep_wide<R, MODE, true, TS, TC>(xr, dr, rr, lane, Pa, T + lb, T + ub);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        if (i < Pa) y[s + i] = (double)xr[r];
+    }
+}
+
 // Partitions larger than a wave: one thread per partition, rows in sequence,
 // summing in the reference's order (left-looking).  Correctness path for
 // large partsize (e.g. partsize = N in the Gauss-Seidel sanity checks).
@@ -673,6 +788,22 @@ static void launch_smooth(const L1GSPlan &h, const TC *D, hipStream_t st, const 
     else launch_seg_smooth<64, TS, TC>(h, D, st, sf);
 }
 
+template <int R, class TS, class TC>
+static void launch_wide(const L1GSPlan &h, const double *x, double *y, const TC *D,
+                        hipStream_t st, const int *done) {
+    const TS *T = reinterpret_cast<const TS *>(h.T.get());
+    const dim3 grid(grid_for(h.nparts, 1)), block(64);
+    if (h.sweep == HS_SWEEP_FORWARD)
+        hipLaunchKernelGGL((k_sweep_wide<R, MODE_FWD, TS, TC>), grid, block, 0, st, x, y, T, D,
+                           h.n, h.P, h.nparts, done);
+    else if (h.sweep == HS_SWEEP_BACKWARD)
+        hipLaunchKernelGGL((k_sweep_wide<R, MODE_BWD, TS, TC>), grid, block, 0, st, x, y, T, D,
+                           h.n, h.P, h.nparts, done);
+    else
+        hipLaunchKernelGGL((k_sweep_wide<R, MODE_SYM, TS, TC>), grid, block, 0, st, x, y, T, D,
+                           h.n, h.P, h.nparts, done);
+}
+
 template <class TS, class TC>
 static void launch_packed(const L1GSPlan &h, const double *x, double *y, const TC *D,
                           hipStream_t st, const int *done) {
@@ -696,6 +827,9 @@ static void launch_packed(const L1GSPlan &h, const double *x, double *y, const T
     else if (h.P <= 16) launch_seg<16, TS, TC>(h, x, y, D, st, done);
     else if (h.P <= 32) launch_seg<32, TS, TC>(h, x, y, D, st, done);
     else if (h.P <= 64) launch_seg<64, TS, TC>(h, x, y, D, st, done);
+    else if (h.P <= 128) launch_wide<2, TS, TC>(h, x, y, D, st, done);
+    else if (h.P <= 192) launch_wide<3, TS, TC>(h, x, y, D, st, done);
+    else if (h.P <= 256) launch_wide<4, TS, TC>(h, x, y, D, st, done);
     else {
         const TS *T = reinterpret_cast<const TS *>(h.T.get());
         unsigned grid = grid_for(h.nparts, 64);

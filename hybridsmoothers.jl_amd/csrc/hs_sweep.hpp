@@ -230,6 +230,7 @@ struct RowAcc;
 template <>
 struct RowAcc<double> {
     double x, acc;
+    __device__ __forceinline__ RowAcc() : x(0.0), acc(0.0) {}
     __device__ __forceinline__ explicit RowAcc(double xr) : x(xr), acc(0.0) {}
     template <bool FAST = true>
     __device__ __forceinline__ double q(double d, double r) const {
@@ -240,6 +241,7 @@ struct RowAcc<double> {
 template <>
 struct RowAcc<float> {
     float rem;
+    __device__ __forceinline__ RowAcc() : rem(0.0f) {}
     __device__ __forceinline__ explicit RowAcc(float xr) : rem(xr) {}
     template <bool FAST = true>
     __device__ __forceinline__ float q(float d, float r) const {

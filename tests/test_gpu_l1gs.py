@@ -96,7 +96,8 @@ def test_eta_nonsym_partsize_known_answers(gpu):
     np.testing.assert_array_equal(P.sweep.op.L.SLbuffer, [-1, 0, -1, -1, 0, -1, -1, 0, -1])
 
 
-@pytest.mark.parametrize("partsize", [1, 2, 3, 5, 10, 16, 17, 31, 32, 33, 63, 64, 65, 100])
+@pytest.mark.parametrize("partsize", [1, 2, 3, 5, 10, 16, 17, 31, 32, 33, 63, 64, 65, 100, 128,
+                                      129, 200, 256, 300])
 @pytest.mark.parametrize("kind", ["sym_csc", "csr", "nonsym_csc"])
 def test_random_parity(gpu, partsize, kind):
     import oracle as O
