@@ -363,6 +363,15 @@ def main():
             variants.append({"name": f"128^3 P={P} symmetric {pname}", "ms": round(m, 5),
                              "bytes": b, "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
             del pv
+        # MatrixViewCache: the matrix read by rows, no packed triangles; bytes
+        # against the CSC-direct layout, (N+1) ix + nnz (ix + 8) + 3 N 8 (Int64)
+        pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(),
+                     cache_strategy=hs.MatrixViewCache())
+        m = time_kernel(lambda: hs.ldiv_(y, pv, x, sync=False), args.steps, args.warmup, stream)
+        b = (n + 1) * 8 + int(A.nnz) * 16 + 3 * n * 8
+        variants.append({"name": f"128^3 P={P} symmetric fp64 MatrixViewCache", "ms": round(m, 5),
+                         "bytes": b, "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
+        del pv
         # partsize 128: one wave per partition, two rows per lane
         pw = builder(A, 128, isSymA=True, sweep=hs.SymmetricSweep())
         m = time_kernel(lambda: hs.ldiv_(y, pw, x, sync=False), args.steps, args.warmup, stream)

@@ -664,6 +664,146 @@ __global__ void k_sweep_view(const double *x, double *y, const int64_t *__restri
             y[g] = (y[g] - view_acc(rowwise, 1, ptr, ind, val, diag, y, g, s, e)) / D[g];
 }
 
+// MatrixViewCache with one wave per partition (partsize <= 64), the matrix
+// read by rows (a CSR, a symmetric CSC read as rows, or the CSR transpose of
+// a non-symmetric CSC).  Lane i owns row g = s + i and the right-looking
+// recurrence runs as in the packed kernels, but a lane adds a term only at
+// the steps where its row has an in-partition entry: it holds the next KV
+// entries of that range (column, value) in registers, compares the next
+// column with the step and, on a match, adds the product and shifts the
+// next entry in (reloading KV more from memory when its cache runs dry).
+// Terms are added in ascending column order going forward and descending
+// going backward -- the packed kernels' order -- and the zeros a packed
+// triangle would add are skipped, which changes nothing for finite values:
+// the results equal PackedBufferCache's bit for bit.
+constexpr int KV = 4;
+
+struct ViewCache {
+    int c[KV];
+    double v[KV];
+};
+
+// loads up to KV entries starting at position p, walking by `dir` (+1 or -1)
+// while inside [lo, hi); columns relative to the partition start s
+__device__ __forceinline__ void view_fill(ViewCache &vc, int64_t p, int dir, int64_t lo, int64_t hi,
+                                          const int64_t *__restrict__ ind,
+                                          const double *__restrict__ val, int64_t s) {
+#pragma unroll
+    for (int k = 0; k < KV; ++k) {
+        const int64_t q = p + dir * k;
+        const bool in = q >= lo && q < hi;
+        vc.c[k] = in ? (int)(ind[q] - 1 - s) : -1;
+        vc.v[k] = in ? val[q] : 0.0;
+    }
+}
+
+// first position in [lo, hi) whose 0-based column is >= c (columns ascend)
+__device__ __forceinline__ int64_t row_lower_bound(const int64_t *__restrict__ ind, int64_t lo,
+                                                   int64_t hi, int64_t c) {
+    while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (ind[mid] - 1 < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int MODE, bool FAST>
+__device__ __forceinline__ double view_wave(double xr, double dr, double rr, int lane, int Pa,
+                                            int64_t s, int64_t plo, int64_t pmid, int64_t pmid1,
+                                            int64_t phi, const int64_t *__restrict__ ind,
+                                            const double *__restrict__ val) {
+    double yr = xr;
+    if constexpr (MODE != MODE_BWD) {
+        // lower entries [plo, pmid): columns s .. g-1, consumed in ascending order
+        RowAcc<double> row(xr);
+        double yfin = 0.0;
+        ViewCache vc;
+        int64_t p = plo;
+        view_fill(vc, p, 1, plo, pmid, ind, val, s);
+        for (int j = 0; j < Pa; ++j) {
+            const double q = row.template q<FAST>(dr, rr);
+            const double yj = seg_bcast<64>(q, j);
+            yfin = (lane == j) ? q : yfin;
+            if (vc.c[0] == j) {
+                row.add(vc.v[0], yj);
+                ++p;
+#pragma unroll
+                for (int k = 0; k + 1 < KV; ++k) {
+                    vc.c[k] = vc.c[k + 1];
+                    vc.v[k] = vc.v[k + 1];
+                }
+                vc.c[KV - 1] = -1;
+                if (vc.c[0] < 0 && p < pmid) view_fill(vc, p, 1, plo, pmid, ind, val, s);
+            }
+        }
+        yr = yfin;
+    }
+    if constexpr (MODE == MODE_SYM) xr = dr * yr;  // y .= D_Dl1 .* y (:897)
+    if constexpr (MODE != MODE_FWD) {
+        // upper entries [pmid1, phi): columns g+1 .. e, consumed in descending order
+        RowAcc<double> row(xr);
+        double yfin = 0.0;
+        ViewCache vc;
+        int64_t p = phi - 1;
+        view_fill(vc, p, -1, pmid1, phi, ind, val, s);
+        for (int j = Pa - 1; j >= 0; --j) {
+            const double q = row.template q<FAST>(dr, rr);
+            const double yj = seg_bcast<64>(q, j);
+            yfin = (lane == j) ? q : yfin;
+            if (vc.c[0] == j) {
+                row.add(vc.v[0], yj);
+                --p;
+#pragma unroll
+                for (int k = 0; k + 1 < KV; ++k) {
+                    vc.c[k] = vc.c[k + 1];
+                    vc.v[k] = vc.v[k + 1];
+                }
+                vc.c[KV - 1] = -1;
+                if (vc.c[0] < 0 && p >= pmid1) view_fill(vc, p, -1, pmid1, phi, ind, val, s);
+            }
+        }
+        yr = yfin;
+    }
+    return yr;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64) void k_sweep_view_wave(const double *x, double *y,
+                                                        const int64_t *__restrict__ ptr,
+                                                        const int64_t *__restrict__ ind,
+                                                        const double *__restrict__ val,
+                                                        const double *__restrict__ D, int64_t n,
+                                                        int64_t P, int64_t nparts,
+                                                        const int *__restrict__ done) {
+    if (done && *done) return;
+    const int64_t k = blockIdx.x;
+    if (k >= nparts) return;
+    const int lane = threadIdx.x;
+    const int64_t s = k * P;
+    const int Pa = (int)(n - s < P ? n - s : P);
+    const bool live = lane < Pa;
+    const int64_t g = s + lane;
+    int64_t plo = 0, pmid = 0, pmid1 = 0, phi = 0;
+    double xr = 0.0, dr = 1.0;
+    if (live) {
+        xr = x[g];
+        dr = D[g];
+        const int64_t p0 = ptr[g] - 1, p1 = ptr[g + 1] - 1;
+        plo = row_lower_bound(ind, p0, p1, s);             // first column >= s
+        pmid = row_lower_bound(ind, plo, p1, g);           // first column >= g
+        pmid1 = (pmid < p1 && ind[pmid] - 1 == g) ? pmid + 1 : pmid;  // past the diagonal
+        phi = row_lower_bound(ind, pmid1, p1, s + Pa);     // first column past the partition
+    }
+    const double rr = 1.0 / dr;
+    double yr;
+    if (__any(!safe_divisor(dr)))
+        yr = view_wave<MODE, false>(xr, dr, rr, lane, Pa, s, plo, pmid, pmid1, phi, ind, val);
+    else
+        yr = view_wave<MODE, true>(xr, dr, rr, lane, Pa, s, plo, pmid, pmid1, phi, ind, val);
+    if (live) y[g] = yr;
+}
+
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
@@ -678,9 +818,12 @@ struct L1GSPlan {
     DBuf<double> D;   // D_Dl1 in FP64 (always kept: getter and FP64 sweeps)
     DBuf<float> Df;   // FP32 copy for the reduced-precision sweeps
     DBuf<uint8_t> T;  // packed triangles (block layout above), element type by `prec`
-    // MatrixViewCache: device copy of the matrix (1-based, as given)
+    // MatrixViewCache: device copy of the matrix (1-based, as given), and for
+    // a non-symmetric CSC its CSR transpose (the rows the wave kernel reads)
     DBuf<int64_t> vptr, vind, vdiag;
     DBuf<double> vval;
+    DBuf<int64_t> wptr, wind;
+    DBuf<double> wval;
     Staging stage;
 
     size_t elem() const { return prec == HS_PREC_FP64 ? 8 : (prec == HS_PREC_FP32 ? 4 : 2); }
@@ -910,7 +1053,22 @@ unsigned l1gs_apply_fused(const L1GSPlan &h, double *z, hipStream_t st, const CG
 void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_t st,
                        const int *done) {
     if (h.n == 0) return;
-    if (h.cache == HS_CACHE_VIEW) {
+    if (h.cache == HS_CACHE_VIEW && h.P <= 64) {
+        // rows: the matrix itself when it is read row-wise, else its transpose
+        const int64_t *rp = h.rowwise ? h.vptr.get() : h.wptr.get();
+        const int64_t *ri = h.rowwise ? h.vind.get() : h.wind.get();
+        const double *rv = h.rowwise ? h.vval.get() : h.wval.get();
+        const dim3 grid(grid_for(h.nparts, 1)), block(64);
+        if (h.sweep == HS_SWEEP_FORWARD)
+            hipLaunchKernelGGL(k_sweep_view_wave<MODE_FWD>, grid, block, 0, st, x, y, rp, ri, rv,
+                               h.D.get(), h.n, h.P, h.nparts, done);
+        else if (h.sweep == HS_SWEEP_BACKWARD)
+            hipLaunchKernelGGL(k_sweep_view_wave<MODE_BWD>, grid, block, 0, st, x, y, rp, ri, rv,
+                               h.D.get(), h.n, h.P, h.nparts, done);
+        else
+            hipLaunchKernelGGL(k_sweep_view_wave<MODE_SYM>, grid, block, 0, st, x, y, rp, ri, rv,
+                               h.D.get(), h.n, h.P, h.nparts, done);
+    } else if (h.cache == HS_CACHE_VIEW) {
         unsigned grid = grid_for(h.nparts, 64);
         const int64_t *dg = h.vdiag.get();
         if (h.sweep == HS_SWEEP_FORWARD)
@@ -941,7 +1099,7 @@ void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_
 // ascending columns inside each row), in O(nnz) instead of O(N^2), on the
 // device.
 static void nonsym_D(L1GSPlan &h, int64_t nnz, const DBuf<int64_t> &ptr, const DBuf<int64_t> &ind,
-                     const DBuf<double> &val, double eta, hipStream_t st) {
+                     const DBuf<double> &val, double eta, hipStream_t st, bool keep_rows) {
     const int64_t n = h.n;
     DBuf<int64_t> trp, tci;
     DBuf<double> tcv;
@@ -950,7 +1108,12 @@ static void nonsym_D(L1GSPlan &h, int64_t nnz, const DBuf<int64_t> &ptr, const D
                        trp.get(), tci.get(), tcv.get(), eta, h.D.get(), (double *)nullptr,
                        h.sweep, 0, 0);
     check_launch();
-    HS_CHECK_HIP(hipStreamSynchronize(st));  // the transpose goes out of scope
+    HS_CHECK_HIP(hipStreamSynchronize(st));
+    if (keep_rows) {  // the MatrixViewCache's row-wise copy
+        h.wptr = std::move(trp);
+        h.wind = std::move(tci);
+        h.wval = std::move(tcv);
+    }
 }
 
 int64_t l1gs_rows(const L1GSPlan &h) { return h.n; }
@@ -976,7 +1139,7 @@ static void build_packed(L1GSPlan &h, int64_t nnz, const DBuf<int64_t> &ptr,
         check_launch();
         return;
     }
-    nonsym_D(h, nnz, ptr, ind, val, eta, st);
+    nonsym_D(h, nnz, ptr, ind, val, eta, st, false);
     hipLaunchKernelGGL((k_pack_cols<TS>), dim3(grid), dim3(256), 0, st, h.n, h.P, ptr.get(),
                        ind.get(), val.get(), T, mode, want_L, want_U);
     check_launch();
@@ -1082,7 +1245,7 @@ int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const doubl
                                        h->D.get(), (double *)nullptr, h->sweep, 0, 0);
                     check_launch();
                 } else {
-                    nonsym_D(*h, nnz, dptr, dind, dval, eta, st);
+                    nonsym_D(*h, nnz, dptr, dind, dval, eta, st, true);
                 }
                 // the view kernels read 0-based diagonal positions
                 h->vptr = std::move(dptr);
@@ -1355,6 +1518,31 @@ int hs_l1gs_load(const char *path, void *stream, hs_l1gs_t *out) {
         up(h->vind, 4);
         up(h->vdiag, 5);
         up(h->vval, 6);
+        if (h->cache == HS_CACHE_VIEW && n > 0) {
+            // the view kernels index through the stored matrix
+            const int64_t nnz = (int64_t)(hd.nbytes[4] / 8);
+            HS_REQUIRE(hd.nbytes[3] == (n + 1) * 8 && hd.nbytes[6] == nnz * 8 &&
+                           (h->rowwise || hd.nbytes[5] == n * 8),
+                       HS_EINVAL, "corrupt L1GS file (matrix sizes)");
+            validate_sparse_device(n, nnz, h->vptr.get(), h->vind.get(), st);
+            if (!h->rowwise) {
+                // diagonal positions recomputed from the validated matrix rather
+                // than trusted from the file; the wave kernel's row-wise copy is
+                // rebuilt, not stored
+                DBuf<unsigned long long> bad(1);
+                unsigned long long init = ~0ull, got = 0;
+                HS_CHECK_HIP(hipMemcpyAsync(bad.get(), &init, 8, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_diag_indices, dim3(grid_for(n, 256)), dim3(256), 0, st, n,
+                                   h->vptr.get(), h->vind.get(), h->vval.get(), h->vdiag.get(),
+                                   bad.get());
+                check_launch();
+                HS_CHECK_HIP(hipMemcpyAsync(&got, bad.get(), 8, hipMemcpyDeviceToHost, st));
+                HS_CHECK_HIP(hipStreamSynchronize(st));
+                HS_REQUIRE(got == ~0ull, HS_EINVAL, "corrupt L1GS file (matrix diagonal)");
+                csc_transpose_device(n, nnz, h->vptr.get(), h->vind.get(), h->vval.get(), h->wptr,
+                                     h->wind, h->wval, st);
+            }
+        }
         HS_CHECK_HIP(hipStreamSynchronize(st));  // the host buffers go out of scope
         *out = h.release();
     });

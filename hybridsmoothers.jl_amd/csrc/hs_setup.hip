@@ -18,13 +18,14 @@ namespace {
 
 constexpr int BLK = 256;
 
-// bad bit 1: pointer array not starting at 1 or decreasing (nnz is read from
-// its last entry); bit 2: an index outside [1, n].
-__global__ void k_check_ptr(int64_t n, const int64_t *__restrict__ ptr1,
+// bad bit 1: pointer array not starting at 1, decreasing or not ending at
+// nnz + 1; bit 2: an index outside [1, n].
+__global__ void k_check_ptr(int64_t n, int64_t nnz, const int64_t *__restrict__ ptr1,
                             unsigned *__restrict__ bad) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const bool ok = (c > 0 || ptr1[0] == 1) && ptr1[c + 1] >= ptr1[c];
+    const bool ok = (c > 0 || ptr1[0] == 1) && ptr1[c + 1] >= ptr1[c] &&
+                    (c < n - 1 || ptr1[n] == nnz + 1);
     if (!ok) atomicOr(bad, 1u);
 }
 
@@ -132,7 +133,7 @@ void validate_sparse_device(int64_t n, int64_t nnz, const int64_t *ptr1, const i
     DBuf<unsigned> bad(1);
     unsigned h = 0;
     HS_CHECK_HIP(hipMemcpyAsync(bad.get(), &h, sizeof h, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_check_ptr, dim3(grid_for(n, BLK)), dim3(BLK), 0, st, n, ptr1,
+    hipLaunchKernelGGL(k_check_ptr, dim3(grid_for(n, BLK)), dim3(BLK), 0, st, n, nnz, ptr1,
                        bad.get());
     if (nnz)
         hipLaunchKernelGGL(k_check_ind, dim3(grid_for(nnz, BLK)), dim3(BLK), 0, st, n, nnz, ind1,
@@ -141,7 +142,8 @@ void validate_sparse_device(int64_t n, int64_t nnz, const int64_t *ptr1, const i
     HS_CHECK_HIP(hipMemcpyAsync(&h, bad.get(), sizeof h, hipMemcpyDeviceToHost, st));
     HS_CHECK_HIP(hipStreamSynchronize(st));
     HS_REQUIRE(!(h & 1u), HS_EINVAL,
-               "invalid sparse matrix: the pointer array must start at 1 and not decrease");
+               "invalid sparse matrix: the pointer array must start at 1, not decrease and end "
+               "at nnz + 1");
     HS_REQUIRE(!(h & 2u), HS_EINVAL, "invalid sparse matrix: an index is outside 1:n");
 }
 

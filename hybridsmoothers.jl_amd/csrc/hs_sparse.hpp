@@ -58,7 +58,7 @@ struct CSR {
 
 // Device setup (hs_setup.hip).  Arrays are 1-based Int64 as SparseMatrixCSC /
 // SparseMatrixCSR hold them.
-// HS_EINVAL unless ptr1 starts at 1 and never decreases (nnz = ptr1[n] - 1)
+// HS_EINVAL unless ptr1 starts at 1, never decreases and ends at nnz + 1,
 // and every index is in [1, n] (device arrays; checked before any other
 // kernel reads them).
 void validate_sparse_device(int64_t n, int64_t nnz, const int64_t *ptr1, const int64_t *ind1,
