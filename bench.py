@@ -324,6 +324,11 @@ def main():
     builder = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=1024)
     log("[bench] L1GS build (P=%d, symmetric, packed FP64)" % P)
     torch.cuda.synchronize()
+    t_cold = time.perf_counter()
+    prec = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep())
+    t_cold = time.perf_counter() - t_cold
+    del prec  # the second build is timed: the first also pays the process's first allocations
+    torch.cuda.synchronize()
     t_build = time.perf_counter()
     prec = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep())
     t_build = time.perf_counter() - t_build
@@ -432,6 +437,7 @@ def main():
         "sweeps": variants,
         "kernel_variant_ms": kernel_variants,
         "build_ms": round(t_build * 1e3, 1),
+        "build_ms_first": round(t_cold * 1e3, 1),
         "operator_build_ms": round(t_op * 1e3, 1),
         "kernel_variant": int(best),
     }
