@@ -17,6 +17,7 @@
 #include "hs_plans.hpp"
 
 #include <cmath>
+#include <cstddef>
 #include <memory>
 
 namespace hs {
@@ -130,6 +131,48 @@ __global__ void k_c4_step(int64_t n, SellView<T> A, const T *__restrict__ zk,
         if (flags & F_ADD_NEXT) xi = xi + (double)zn;
     }
     x[i] = xi;
+}
+
+// ---- 4th kind inside the fused CG iteration --------------------------------
+// init, preconditioner mode: the CG updates r -= alpha q, x += alpha p
+// (cg_prologue, the k_cg_axpy arithmetic) and then z1 = b1 l1inv r
+template <class T>
+__global__ void k_c4_init_cg(int64_t n, const T *__restrict__ l1inv, T *__restrict__ z1, T a1,
+                             T b1, CGFuse f) {
+    if (f.st->done) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T ri = (T)cg_prologue(f, i);
+    z1[i] = a1 * (T)0 + b1 * l1inv[i] * ri;
+}
+
+// the last step of the preconditioner (degree d: step d-1, which also adds
+// z_d) with the r'z partial of its 256-row block, as k_cg_dot forms it
+template <class T, class TR>
+__global__ __launch_bounds__(256) void k_c4_step_dot(int64_t n, SellView<T> A,
+                                                     const T *__restrict__ zk,
+                                                     const TR *__restrict__ rin,
+                                                     const T *__restrict__ l1inv,
+                                                     double *__restrict__ x, T an, T bn, int flags,
+                                                     const double *__restrict__ rcg,
+                                                     double *__restrict__ partials,
+                                                     const int *__restrict__ done) {
+    __shared__ double sm[4];
+    if (*done) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const T ri = row_sub(A, zk, i, (T)rin[i]);
+        const T zi = zk[i];
+        double xi = (flags & F_X_ZERO) ? 0.0 : x[i];
+        xi = xi + (double)zi;
+        const T zn = an * zi + bn * l1inv[i] * ri;
+        xi = xi + (double)zn;
+        x[i] = xi;
+        v = rcg[i] * xi;
+    }
+    v = block_sum(v, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
 // ---- 1st kind -------------------------------------------------------------
@@ -319,6 +362,66 @@ static void cheb_run(ChebPlan &s, double *x, const double *b, bool as_prec, hipS
         rs = rsn;
     }
     s.zcur = cur;
+}
+
+// The 4th-kind preconditioner (degree >= 2) inside the fused CG iteration:
+// same launches and arithmetic as cheb_run in preconditioner mode, with the
+// CG's r and x updates in the init kernel and r'z in the last step.
+template <class T>
+static unsigned cheb_run_cg(ChebPlan &s, double *z, hipStream_t st, const CGFuse &f) {
+    const int64_t n = s.n;
+    const unsigned g = grid_for(n, 256);
+    const SellView<T> Av = s.A.template view_as<T>();
+    const T *l1;
+    if constexpr (sizeof(T) == 8) l1 = s.l1inv.get();
+    else l1 = s.l1inv32.get();
+    T *r = s.ptr<T>(s.r);
+    T *zb[2] = {s.ptr<T>(s.za), s.ptr<T>(s.zb)};
+    const int d = s.degree;
+    const int *done = reinterpret_cast<const int *>(reinterpret_cast<const char *>(f.st) +
+                                                    offsetof(CGDev, done));
+    auto ab = [&](int k, T &ak, T &bk) {
+        ak = (T)((double)(2 * k - 3) / (double)(2 * k + 1));
+        bk = (T)((double)(8 * k - 4) / ((double)(2 * k + 1) * s.rho));
+    };
+    T a1, b1;
+    ab(1, a1, b1);
+    hipLaunchKernelGGL((k_c4_init_cg<T>), dim3(g), dim3(256), 0, st, n, l1, zb[0], a1, b1, f);
+    check_launch();
+    int cur = 0;
+    for (int k = 1; k <= d - 1; ++k) {
+        const bool last = (k == d - 1);
+        T an, bn;
+        ab(k + 1, an, bn);
+        int fl = F_HAS_NEXT | (k == 1 ? F_X_ZERO : 0) | (last ? F_ADD_NEXT : F_WRITE_R);
+        if (!last) {
+            if (k == 1)
+                hipLaunchKernelGGL((k_c4_step<T, double>), dim3(g), dim3(256), 0, st, n, Av,
+                                   zb[cur], (const double *)f.r, r, l1, z, zb[cur ^ 1], an, bn, fl,
+                                   done);
+            else
+                hipLaunchKernelGGL((k_c4_step<T, T>), dim3(g), dim3(256), 0, st, n, Av, zb[cur],
+                                   r, r, l1, z, zb[cur ^ 1], an, bn, fl, done);
+            cur ^= 1;
+        } else if (k == 1) {
+            hipLaunchKernelGGL((k_c4_step_dot<T, double>), dim3(g), dim3(256), 0, st, n, Av,
+                               zb[cur], (const double *)f.r, l1, z, an, bn, fl, f.r, f.partials,
+                               done);
+        } else {
+            hipLaunchKernelGGL((k_c4_step_dot<T, T>), dim3(g), dim3(256), 0, st, n, Av, zb[cur],
+                               r, l1, z, an, bn, fl, f.r, f.partials, done);
+        }
+        check_launch();
+    }
+    s.zcur = cur;
+    return g;
+}
+
+bool cheb_can_fuse(const ChebPlan &s) { return s.kind == 4 && s.degree >= 2 && s.n > 0; }
+
+unsigned cheb_apply_fused(ChebPlan &s, double *z, hipStream_t st, const CGFuse &f) {
+    if (s.prec == HS_PREC_FP64) return cheb_run_cg<double>(s, z, st, f);
+    return cheb_run_cg<float>(s, z, st, f);
 }
 
 void cheb_apply_device(ChebPlan &s, double *x, const double *b, bool as_prec, hipStream_t st,

@@ -260,12 +260,14 @@ static void apply_prec(const PrecRef &M, const double *r, double *z, int64_t n, 
     }
 }
 
-// One iteration of the fused form: 4 launches, see k_cg_spmv_fused and CGFuse.
+// One iteration of the fused form: SpMV, finalize delta, the preconditioner
+// with the CG updates folded in (an L1GS sweep: one launch; a 4th-kind
+// Chebyshev smoother of degree d: d launches), finalize gamma.  See
+// k_cg_spmv_fused and CGFuse.
 static void enqueue_iteration_fused(const CSR &A, const PrecRef &M, CGWork &w, hipStream_t st) {
     const int64_t n = w.n;
     const unsigned g = grid_for(n, 256);
     CGDev *d = w.st.get();
-    const L1GSPlan &plan = *reinterpret_cast<const L1GSPlan *>(M.plan);
     hipLaunchKernelGGL(k_cg_spmv_fused, dim3(g), dim3(256), 0, st, n, A.view(), w.z.get(),
                        w.p.get(), w.p2.get(), w.q.get(), w.partials.get(), d);
     hipLaunchKernelGGL(k_cg_fin_delta, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), g, d);
@@ -274,7 +276,10 @@ static void enqueue_iteration_fused(const CSR &A, const PrecRef &M, CGWork &w, h
     // replays fixed pointers, so both buffers are passed and the parity of
     // st->iter picks one on the device
     CGFuse f{w.r.get(), w.q.get(), w.p.get(), w.p2.get(), w.x.get(), w.partials.get(), d};
-    unsigned cnt = l1gs_apply_fused(plan, w.z.get(), st, f);
+    const unsigned cnt =
+        M.kind == HS_PCG_L1GS
+            ? l1gs_apply_fused(*reinterpret_cast<const L1GSPlan *>(M.plan), w.z.get(), st, f)
+            : cheb_apply_fused(*reinterpret_cast<ChebPlan *>(M.plan), w.z.get(), st, f);
     hipLaunchKernelGGL(k_cg_fin_gamma, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), cnt, d);
     check_launch();
 }
@@ -327,8 +332,9 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
     check_launch();
 
     if (block < 1) block = 1;
-    const bool fused = M.kind == HS_PCG_L1GS &&
-                       l1gs_can_fuse(*reinterpret_cast<const L1GSPlan *>(M.plan));
+    const bool fused =
+        (M.kind == HS_PCG_L1GS && l1gs_can_fuse(*reinterpret_cast<const L1GSPlan *>(M.plan))) ||
+        (M.kind == HS_PCG_CHEB && cheb_can_fuse(*reinterpret_cast<const ChebPlan *>(M.plan)));
     auto iteration = [&](hipStream_t s) {
         if (fused) enqueue_iteration_fused(A, M, w, s);
         else enqueue_iteration(A, M, w, s);

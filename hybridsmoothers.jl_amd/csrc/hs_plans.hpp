@@ -126,6 +126,17 @@ struct CGFuse {
     const CGDev *st;
 };
 
+// Row prologue of the fused CG form: r -= alpha q; x += alpha p (the k_cg_axpy
+// arithmetic); returns the updated r, the preconditioner's input.
+__device__ __forceinline__ double cg_prologue(const CGFuse &f, int64_t i) {
+    const double alpha = f.st->gamma / f.st->delta;
+    const double *p = (f.st->iter & 1) ? f.pb : f.pa;
+    const double ri = f.r[i] - alpha * f.q[i];
+    f.r[i] = ri;
+    f.x[i] = f.x[i] + alpha * p[i];
+    return ri;
+}
+
 // The fused smoother form, one round of AMG.smooth! (ext/HybridSmoothersAMGExt.jl:
 // 29-35): r_i = (A x)_i - b_i computed in the sweep's prologue (row sum in
 // column order, then the subtraction, as mul! followed by .-= rounds them),
@@ -152,6 +163,11 @@ bool l1gs_can_fuse(const L1GSPlan &h);
 unsigned l1gs_apply_fused(const L1GSPlan &h, double *z, hipStream_t st, const CGFuse &f);
 int64_t l1gs_rows(const L1GSPlan &h);
 uint64_t l1gs_serial(const L1GSPlan &h);
+
+// The 4th-kind smoother (degree >= 2) as the fused CG's preconditioner: z =
+// M^-1 (r - alpha q) with the CGFuse updates; returns the r'z partial count.
+bool cheb_can_fuse(const ChebPlan &s);
+unsigned cheb_apply_fused(ChebPlan &s, double *z, hipStream_t st, const CGFuse &f);
 
 // Chebyshev smoother: smooth!(x, s, b), or z = 0; smooth!(z, s, b) when as_prec.
 void cheb_apply_device(ChebPlan &s, double *x, const double *b, bool as_prec, hipStream_t st,

@@ -510,16 +510,6 @@ __device__ __forceinline__ TC sweep_full_symt64(TC xr, TC dr, TC rr, int lane,
     return (lane == 0) ? q : yfin;
 }
 
-// Row prologue of the fused CG form: r -= alpha q; x += alpha p (the k_cg_axpy
-// arithmetic); returns the updated r, the sweep's input.
-__device__ __forceinline__ double cg_prologue(const CGFuse &f, int64_t i) {
-    const double alpha = f.st->gamma / f.st->delta;
-    const double *p = (f.st->iter & 1) ? f.pb : f.pa;
-    const double ri = f.r[i] - alpha * f.q[i];
-    f.r[i] = ri;
-    f.x[i] = f.x[i] + alpha * p[i];
-    return ri;
-}
 
 
 // One partition per PM-lane segment; returns x_i * y_i of the lane's row

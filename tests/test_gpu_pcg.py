@@ -66,7 +66,8 @@ def test_c1_readme_quickstart(gpu):
     assert sol.true_relres < 1e-6
 
 
-@pytest.mark.parametrize("prec", ["l1gs64", "l1gs32_fwd", "cheb4_3", "cheb1_2", "cheb4_1"])
+@pytest.mark.parametrize("prec", ["l1gs64", "l1gs32_fwd", "cheb4_3", "cheb4_2", "cheb4_5",
+                                  "cheb1_2", "cheb4_1"])
 def test_preconditioned_parity(gpu, prec):
     import oracle as O
     from hybridsmoothers import problems as pb
