@@ -413,6 +413,64 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
     (void)sweep_seg_body<PM, MODE, TS, TC, FORM>(x, y, T, D, n, P, nparts, k, lane, f, sf);
 }
 
+// One-sided sweeps with P == PM (16 or 32): each wave first copies the packed
+// blocks of its 64 / PM partitions -- consecutive in the buffer -- into LDS
+// with full-wave loads (NCH aligned 64-element chunks), then runs the same
+// recurrence reading its columns from LDS.  Loading columns straight from
+// memory issues one partly overlapping wave load per step; this issues
+// (64/PM) P(P-1)/2 / 64 aligned ones.  A wave holding the last (short)
+// partition keeps the direct path.
+template <int PM, int MODE, class TS, class TC>
+__global__ __launch_bounds__(256) void k_sweep_seg_lds(const double *x, double *y,
+                                                       const TS *__restrict__ T,
+                                                       const TC *__restrict__ D, int64_t n, int P,
+                                                       int64_t nparts, const int *__restrict__ done) {
+    static_assert(MODE == MODE_FWD || MODE == MODE_BWD, "one triangle per block");
+    if (done && *done) return;
+    constexpr int SW = 64 / PM;             // partitions per wave
+    constexpr int BLK = PM * (PM - 1) / 2;  // elements per block
+    constexpr int WE = SW * BLK;            // elements per wave
+    constexpr int NCH = (WE + 63) / 64;
+    __shared__ TS lds_all[4 * NCH * 64];
+    const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
+    const int lane = l64 % PM, seg = l64 / PM;
+    const int64_t kw = ((int64_t)blockIdx.x * 4 + wave) * SW;  // the wave's first partition
+    const int64_t k = kw + seg;
+    if (__all(k >= nparts)) return;
+    const bool staged = (kw + SW) * (int64_t)PM <= n;  // the same in every lane of the wave
+    if (!staged) {
+        (void)sweep_seg_body<PM, MODE, TS, TC, FORM_PLAIN>(x, y, T, D, n, P, nparts, k, lane,
+                                                           CGFuse{}, SmoothFuse{});
+        return;
+    }
+    TS *const lw = lds_all + wave * NCH * 64;
+    const TS *const src = T + kw * (int64_t)BLK;
+    TS chunk[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int idx = c * 64 + l64;
+        chunk[c] = src[idx < WE ? idx : WE - 1];
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int idx = c * 64 + l64;
+        if (idx < WE) lw[idx] = chunk[c];
+    }
+    // the stores above before other lanes' reads below
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int64_t g = k * PM + lane;
+    const TC xr = (TC)x[g];
+    const TC dr = D[g];
+    const TC rr = (TC)1 / dr;
+    const TS *const blk = lw + seg * BLK;
+    TC yr;
+    if (__any(!safe_divisor(dr)))
+        yr = sweep_generic<PM, MODE, false, TS, TC>(xr, dr, rr, lane, PM, true, blk, blk);
+    else
+        yr = sweep_full<PM, MODE, 16, TS, TC>(xr, dr, rr, lane, blk, blk);
+    y[g] = (double)yr;
+}
+
 // One-triangle symmetric sweep (HS_CACHE_PACKED_SYM): one wave per workgroup,
 // 64 / PM partitions per wave, each with its LDS copy of L (symt_lds(PM)
 // elements: 17 KB per wave in FP64 at PM = 64), so that LDS does not limit
@@ -842,6 +900,17 @@ static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *
     unsigned grid = grid_for(h.nparts, SEGS);
     const TS *T = reinterpret_cast<const TS *>(h.T.get());
     int P = (int)h.P;
+    if constexpr (PM == 16 || PM == 32) {
+        if (P == PM && h.sweep != HS_SWEEP_SYMMETRIC) {
+            if (h.sweep == HS_SWEEP_FORWARD)
+                hipLaunchKernelGGL((k_sweep_seg_lds<PM, MODE_FWD, TS, TC>), dim3(grid), dim3(256),
+                                   0, st, x, y, T, D, h.n, P, h.nparts, done);
+            else
+                hipLaunchKernelGGL((k_sweep_seg_lds<PM, MODE_BWD, TS, TC>), dim3(grid), dim3(256),
+                                   0, st, x, y, T, D, h.n, P, h.nparts, done);
+            return;
+        }
+    }
     switch (h.sweep) {
     case HS_SWEEP_FORWARD:
         hipLaunchKernelGGL((k_sweep_seg<PM, MODE_FWD, TS, TC>), dim3(grid), dim3(256), 0, st, x, y,
