@@ -286,3 +286,31 @@ def test_view_wave_equals_packed(gpu, kind, partsize, density):
         Pv = b(M, partsize, isSymA=is_sym, sweep=_hs_sweep(hs, sw),
                cache_strategy=hs.MatrixViewCache())
         np.testing.assert_array_equal(hs.backslash(Pv, x), hs.backslash(Pp, x))
+
+
+def test_c3_convdiff2048_parity(gpu):
+    """2048^2 upwind convection-diffusion (4.19 M rows, non-symmetric CSC),
+    DiagonalIndices path, P=32 forward and backward -- full size: D_Dl1 and
+    the forward sweep bit-exact with the oracle, the backward within 1e-12."""
+    import torch
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.convdiff2d(2048)
+    n = A.shape[0]
+    x = pb.uniform_rhs(n, 1)
+    xd = torch.tensor(x, device="cuda")
+    yd = torch.empty_like(xd)
+    b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=1024)
+    for sw in ("forward", "backward"):
+        ref = O.l1gs_build(A, 32, is_sym=False, sweep=sw, gpu=(256, 1024))
+        P = b(A, 32, isSymA=False, sweep=_hs_sweep(hs, sw))
+        op = P.sweep.op
+        np.testing.assert_array_equal(op.D_DL1, ref.D)
+        hs.ldiv_(yd, P, xd)
+        y = yd.cpu().numpy()
+        yo = O.l1gs_ldiv(ref, x)
+        if sw == "forward":
+            np.testing.assert_array_equal(y, yo)
+        else:
+            assert relerr(y, yo) < TOL
