@@ -23,9 +23,10 @@
 //   k_b_fin_gamma systems: the CG scalars and the stopping test
 // The sweep is persistent: a fixed grid, each workgroup walking a contiguous
 // run of the tile table (neighbouring tiles of a system share the workgroup's
-// XCD and its L2); the SpMV takes one tile per workgroup of a large grid.  A fixed number of passes is captured once in
-// a HIP graph; admission and retirement only rewrite the device tables
-// between graph launches, so the graph is never re-captured.
+// XCD and its L2); the SpMV takes one tile per workgroup of a large grid.  A
+// fixed number of passes is captured once per SpMV grid size (SPMV_GRIDS) in
+// HIP graphs; admission and retirement only rewrite the device tables between
+// graph launches, so no graph is ever re-captured.
 //
 // Every per-system reduction sums that system's own tile partials in a fixed
 // order, so a system's result does not depend on which others share the
@@ -44,7 +45,12 @@ namespace hs {
 namespace batch {
 
 constexpr int TILE = 256;         // rows per tile (= one 256-thread workgroup)
-constexpr int SPMV_GRID = 65536;  // workgroups of the SpMV launch (16.8 M rows per pass)
+// Workgroups of the SpMV launch: one graph is captured per grid size and a
+// step replays the smallest that covers the tile table (past 16.8 M rows the
+// largest strides).  A workgroup past the tile count exits at once, but
+// dispatching 65,536 of them still costs about 14 us per launch, which a
+// nearly empty engine would pay on every pass.
+constexpr int SPMV_GRIDS[3] = {1024, 8192, 65536};
 
 enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_RESID = 3, PH_DONE = 4 };
 
@@ -516,7 +522,8 @@ struct Engine {
     std::vector<int> activated;  // slots whose descriptor must be uploaded
     bool tables_dirty = false;
     int32_t nactive = 0;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};  // by SpMV grid, SPMV_GRIDS order
+    int32_t ntiles_host = 0;
     hipEvent_t ev_step = nullptr;
     bool step_pending = false;
     int64_t next_id = 0, steps = 0, retired = 0;
@@ -528,7 +535,7 @@ struct Engine {
     }
     ~Engine();
     void create(const hs_batch_config &c);
-    void enqueue_pass(hipStream_t s);
+    void enqueue_pass(hipStream_t s, unsigned spmv_grid);
     void capture();
     int submit(int64_t n, const int64_t *ptr, const int64_t *ind, const double *val,
                const double *b, double *x_out, double rtol, double atol, int64_t itmax,
@@ -541,7 +548,8 @@ struct Engine {
 Engine::~Engine() {
     if (cs) (void)hipStreamSynchronize(cs);
     if (us) (void)hipStreamSynchronize(us);
-    if (graph) (void)hipGraphExecDestroy(graph);
+    for (hipGraphExec_t g : graphs)
+        if (g) (void)hipGraphExecDestroy(g);
     for (auto &s : slots)
         if (s.ready) (void)hipEventDestroy(s.ready);
     if (ev_step) (void)hipEventDestroy(ev_step);
@@ -605,8 +613,7 @@ void Engine::create(const hs_batch_config &c) {
     dsz = c.precision == HS_PREC_FP64 ? 8 : 4;
     const bool fits = symt_fits_rt((int)c.partsize, c.precision);
     HS_REQUIRE(c.triangles != 1 || fits, HS_EUNSUPPORTED,
-               "one-triangle storage needs partsize 16, partsize 32 in FP32/BF16 or "
-               "partsize 64 in BF16");
+               "one-triangle storage needs partsize 16, or partsize 32 or 64 in FP32/BF16");
     one_tri = c.triangles == 1 || (c.triangles == 0 && fits);
     int dev = 0;
     HS_CHECK_HIP(hipGetDevice(&dev));
@@ -616,7 +623,7 @@ void Engine::create(const hs_batch_config &c) {
     // the chip at once (each walks a fixed run of the tile table; a second
     // round of workgroups would wait for the first to finish).
     const int cus = prop.multiProcessorCount;
-    grid_spmv = SPMV_GRID;
+    grid_spmv = SPMV_GRIDS[2];
     grid_sweep = c.grid > 0 ? c.grid
                             : resident_grid(sweep_fn((int)c.partsize, c.precision, one_tri), cus);
     HS_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -682,13 +689,13 @@ static void launch_sweep(int prec, bool symt, unsigned grid, hipStream_t s, cons
         launch_sweep_t<PM, __hip_bfloat16, float>(symt, grid, s, tiles, cnt, sys, st, arena);
 }
 
-void Engine::enqueue_pass(hipStream_t s) {
+void Engine::enqueue_pass(hipStream_t s, unsigned spmv_grid) {
     const unsigned fin_grid = (unsigned)std::min<int64_t>(cfg.max_systems, 256);
     const Tile *tl = d_tiles.get();
     const Counts *cn = d_cnt.get();
     const Sys *sy = d_sys.get();
     State *stt = d_st.get();
-    hipLaunchKernelGGL(k_b_spmv, dim3(grid_spmv), dim3(TILE), 0, s, tl, cn, sy, stt, arena_dev);
+    hipLaunchKernelGGL(k_b_spmv, dim3(spmv_grid), dim3(TILE), 0, s, tl, cn, sy, stt, arena_dev);
     hipLaunchKernelGGL(k_b_fin_delta, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(), cn, sy,
                        stt, arena_dev);
     if (cfg.partsize == 16)
@@ -703,12 +710,14 @@ void Engine::enqueue_pass(hipStream_t s) {
 }
 
 void Engine::capture() {
-    hipGraph_t g;
-    HS_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < cfg.passes; ++k) enqueue_pass(cs);
-    HS_CHECK_HIP(hipStreamEndCapture(cs, &g));
-    HS_CHECK_HIP(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
+    for (int v = 0; v < 3; ++v) {
+        hipGraph_t g;
+        HS_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < cfg.passes; ++k) enqueue_pass(cs, (unsigned)SPMV_GRIDS[v]);
+        HS_CHECK_HIP(hipStreamEndCapture(cs, &g));
+        HS_CHECK_HIP(hipGraphInstantiate(&graphs[v], g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+    }
 }
 
 int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const double *val,
@@ -819,6 +828,7 @@ void Engine::step() {
         h_cnt->ntiles = nt;
         h_cnt->nactive = na;
         nactive = na;
+        ntiles_host = nt;
         for (int s : activated) {
             const Slot &S = slots[s];
             const Layout &L = S.lay;
@@ -861,7 +871,10 @@ void Engine::step() {
         tables_dirty = false;
     }
     if (nactive == 0) return;
-    HS_CHECK_HIP(hipGraphLaunch(graph, cs));
+    int v = 0;
+    while (v < 2 && SPMV_GRIDS[v] < ntiles_host) ++v;
+    grid_spmv = SPMV_GRIDS[v];
+    HS_CHECK_HIP(hipGraphLaunch(graphs[v], cs));
     HS_CHECK_HIP(hipMemcpyAsync(h_out, d_st.get(), slots.size() * sizeof(State),
                                 hipMemcpyDeviceToHost, cs));
     HS_CHECK_HIP(hipEventRecord(ev_step, cs));

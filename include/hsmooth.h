@@ -212,10 +212,10 @@ typedef struct {
     int precision;          /* smoother storage: HS_PREC_FP64, HS_PREC_FP32 or HS_PREC_BF16 */
     int passes;             /* CG iterations per step (one graph launch) */
     int grid;               /* workgroups of the persistent sweep kernel; 0 = as many as are
-                               resident at once (the SpMV always uses one per 256-row tile) */
+                               resident at once (the SpMV uses one per 256-row tile) */
     double eta;             /* the l1 rule's eta (reference default 1.5) */
     int triangles;          /* packed triangles stored per partition: 1 the lower only (U = L^T,
-                               half the bytes; partsize 16, 32 in FP32/BF16, 64 in BF16),
+                               half the bytes; partsize 16, 32 and 64 in FP32/BF16),
                                2 both, 0 automatic (1 where possible) */
 } hs_batch_config;
 
@@ -250,9 +250,10 @@ int hs_batch_step(hs_batch_t e);
 int hs_batch_collect(hs_batch_t e, hs_batch_result *out, int64_t max, int64_t *count);
 /* out[0..11] = active, uploading, free slots, free arena bytes, arena bytes,
  * largest free arena block, steps, retired, results waiting, the workgroups
- * of the SpMV and sweep tile kernels, and the packed triangles stored per
+ * of the SpMV (in the last step: the smallest of 1024, 8192, 65536 covering
+ * the tile table) and sweep tile kernels, and the packed triangles stored per
  * partition (1: the lower one only -- U = L^T -- whenever the sweep's LDS copy
- * of L fits: partsize 16, partsize 32 in FP32/BF16, partsize 64 in BF16; 2: both) */
+ * of L fits: partsize 16, partsize 32 and 64 in FP32/BF16; 2: both) */
 int hs_batch_info(hs_batch_t e, int64_t *out);
 /* Copies a vector of a resident (submitted, not yet collected) system to host
  * `out` after the last step: what = 0 x, 1 r, 2 z, 3 b, 4 q (n doubles each),
