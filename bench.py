@@ -70,6 +70,30 @@ def time_kernel(fn, steps, warmup, stream):
     return start.elapsed_time(end) / steps  # ms per launch
 
 
+def time_launches(fn, steps, warmup, stream, flush=None):
+    """Per-launch times (ms): p10, p50, p90 over `steps` launches, each between
+    its own pair of events.  With `flush` (a device tensor, >= 512 MB), the
+    tensor is read (summed) before every launch, outside the events, so the
+    launch finds the 256 MB Infinity Cache holding other, clean data and its
+    working set starts from HBM."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    for a, b in ev:
+        if flush is not None:
+            flush.sum()
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    t = np.array([a.elapsed_time(b) for a, b in ev])
+    return {"p10": round(float(np.percentile(t, 10)), 5), "p50": round(float(np.median(t)), 5),
+            "p90": round(float(np.percentile(t, 90)), 5)}
+
+
 def pcg_bench(hs, M, b, P, label, reps=3):
     """Time `reps` PCG solves (x0 = 0, rtol 1e-8) after one warm-up solve."""
     sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8)
@@ -356,44 +380,50 @@ def main():
     prec.set_variant(int(best))
     ms = time_kernel(step, args.steps, args.warmup, stream)
     gbs = nbytes / (ms * 1e-3) / 1e9
+    launch_ms = time_launches(step, min(args.steps, 100), args.warmup, stream)
     variants = []
+    # working sets under 512 MB are also timed launch by launch with the
+    # Infinity Cache flushed in between ("flushed_p50_ms", GB/s from it)
+    flush = torch.zeros(512 << 17, dtype=torch.float64, device="cuda") if not args.quick else None
+
+    def add_variant(name, fn, b, **extra):
+        m = time_kernel(fn, args.steps, args.warmup, stream)
+        row = {"name": name, "ms": round(m, 5), "bytes": b, "GB/s": round(b / (m * 1e-3) / 1e9, 1)}
+        if b < (512 << 20):
+            f = time_launches(fn, min(args.steps, 100), args.warmup, stream, flush)["p50"]
+            row["flushed_p50_ms"] = f
+            row["flushed_GB/s"] = round(b / (f * 1e-3) / 1e9, 1)
+        row.update(extra)
+        variants.append(row)
+
     if not args.quick:
         # reduced-precision smoothers on the same matrix (FP64 x in, FP64 y out)
         for pname, tb, db in (("fp32", 4, 4), ("bf16", 2, 4)):
             pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname)
             pv.set_variant(int(best))
-            m = time_kernel(lambda: hs.ldiv_(y, pv, x, sync=False), args.steps, args.warmup,
-                            stream)
-            b = sweep_bytes(n, S, tri_bytes=tb, d_bytes=db)
-            variants.append({"name": f"128^3 P={P} symmetric {pname}", "ms": round(m, 5),
-                             "bytes": b, "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
+            add_variant(f"128^3 P={P} symmetric {pname}", lambda: hs.ldiv_(y, pv, x, sync=False),
+                        sweep_bytes(n, S, tri_bytes=tb, d_bytes=db))
             del pv
         # MatrixViewCache: the matrix read by rows, no packed triangles; bytes
         # against the CSC-direct layout, (N+1) ix + nnz (ix + 8) + 3 N 8 (Int64)
         pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(),
                      cache_strategy=hs.MatrixViewCache())
-        m = time_kernel(lambda: hs.ldiv_(y, pv, x, sync=False), args.steps, args.warmup, stream)
-        b = (n + 1) * 8 + int(A.nnz) * 16 + 3 * n * 8
-        variants.append({"name": f"128^3 P={P} symmetric fp64 MatrixViewCache", "ms": round(m, 5),
-                         "bytes": b, "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
+        add_variant(f"128^3 P={P} symmetric fp64 MatrixViewCache",
+                    lambda: hs.ldiv_(y, pv, x, sync=False),
+                    (n + 1) * 8 + int(A.nnz) * 16 + 3 * n * 8)
         del pv
         # partsize 128: one wave per partition, two rows per lane
         pw = builder(A, 128, isSymA=True, sweep=hs.SymmetricSweep())
-        m = time_kernel(lambda: hs.ldiv_(y, pw, x, sync=False), args.steps, args.warmup, stream)
-        b = sweep_bytes(n, pw.packed_size)
-        variants.append({"name": "128^3 P=128 symmetric fp64", "ms": round(m, 5), "bytes": b,
-                         "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
+        add_variant("128^3 P=128 symmetric fp64", lambda: hs.ldiv_(y, pw, x, sync=False),
+                    sweep_bytes(n, pw.packed_size))
         del pw
         # one stored triangle (U = L^T): same results, half the packed bytes
         for pname, tb, db in (("fp64", 8, 8), ("fp32", 4, 4), ("bf16", 2, 4)):
             pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname,
                          cache_strategy=hs.PackedSymmetricCache())
-            m = time_kernel(lambda: hs.ldiv_(y, pv, x, sync=False), args.steps, args.warmup,
-                            stream)
-            b = sweep_bytes(n, S, tri_bytes=tb, d_bytes=db, tris=1)
-            variants.append({"name": f"128^3 P={P} symmetric {pname} one-triangle",
-                             "ms": round(m, 5), "bytes": b,
-                             "GB/s": round(b / (m * 1e-3) / 1e9, 1)})
+            add_variant(f"128^3 P={P} symmetric {pname} one-triangle",
+                        lambda: hs.ldiv_(y, pv, x, sync=False),
+                        sweep_bytes(n, S, tri_bytes=tb, d_bytes=db, tris=1))
             del pv
         # non-symmetric 2D upwind convection-diffusion 2048^2, P=32, DiagonalIndices path
         log("[bench] building 2048^2 convection-diffusion")
@@ -405,13 +435,9 @@ def main():
             tb = time.perf_counter()
             p3 = builder(A3, 32, isSymA=False, sweep=sw)
             tb = time.perf_counter() - tb
-            m = time_kernel(lambda: hs.ldiv_(y3, p3, x3, sync=False), args.steps, args.warmup,
-                            stream)
-            b = p3.packed_size * 8 + 3 * n3 * 8
-            variants.append({"name": f"{args.grid2d}^2 convdiff P=32 {type(sw).__name__}",
-                             "ms": round(m, 5), "bytes": b,
-                             "GB/s": round(b / (m * 1e-3) / 1e9, 1),
-                             "build_ms": round(tb * 1e3, 1)})
+            add_variant(f"{args.grid2d}^2 convdiff P=32 {type(sw).__name__}",
+                        lambda: hs.ldiv_(y3, p3, x3, sync=False), p3.packed_size * 8 + 3 * n3 * 8,
+                        build_ms=round(tb * 1e3, 1))
             del p3
     out = {
         "metric": "L1GS sweep GB/s (% HBM peak) at 128³; batched PCG solves/s + p99 at rtol 1e-8",
@@ -436,6 +462,7 @@ def main():
                      "algorithmic_bytes": nbytes},
         "sweeps": variants,
         "kernel_variant_ms": kernel_variants,
+        "launch_ms": launch_ms,
         "build_ms": round(t_build * 1e3, 1),
         "build_ms_first": round(t_cold * 1e3, 1),
         "operator_build_ms": round(t_op * 1e3, 1),
