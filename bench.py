@@ -127,9 +127,23 @@ def cpu_baseline(A, partsize, budget_s=12.0):
     dt = (time.perf_counter() - t0) / reps
     S = ref.SL.size
     gbs = sweep_bytes(n, S) / dt / 1e9
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} symmetric L1GS applications of the 128^3 problem (P={partsize}, "
-                      f"FP64 packed), oracle restatement with OpenMP, {dt * 1e3:.1f} ms each"}
+    out = {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+           "sample": f"{reps} symmetric L1GS applications of the 128^3 problem (P={partsize}, "
+                     f"FP64 packed), oracle restatement with OpenMP, {dt * 1e3:.1f} ms each"}
+    # the CPU default layout (MatrixViewCache, :613), a few applications
+    view = O.l1gs_build(A, partsize, is_sym=True, sweep="symmetric", cache="view", chunks=threads,
+                        nthreads=threads)
+    O.l1gs_ldiv(view, x)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        O.l1gs_ldiv(view, x)
+    out["view_ms_per_apply"] = round((time.perf_counter() - t0) / 10 * 1e3, 2)
+    # one PCG solve to rtol 1e-8 with the packed preconditioner (same restated CG)
+    t0 = time.perf_counter()
+    _, st = O.pcg(A, x, ref, rtol=1e-8, nthreads=threads)
+    out["pcg_ms_per_solve"] = round((time.perf_counter() - t0) * 1e3, 1)
+    out["pcg_iters"] = st.iters
+    return out
 
 
 def serving_bench(args, log_fn=log):
