@@ -167,11 +167,12 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned G) {
 }
 
 // Thread t adds partials t, t + blockDim, t + 2 blockDim, ... one after the
-// other; the loads of 8 consecutive terms are issued before their additions
-// (same order, same result), so the loop is not one load latency per term.
+// other; the loads of 32 consecutive terms are issued before their additions
+// (same order, same result), so the loop is not one load latency per term
+// (32,768 partials over 1,024 threads: one round of loads).
 __device__ __forceinline__ double finalize_sum(const double *__restrict__ partials, unsigned count,
                                               unsigned stride, double *smem16) {
-    constexpr unsigned U = 8;
+    constexpr unsigned U = 32;
     const unsigned step = blockDim.x;
     double v = 0.0;
     unsigned b = threadIdx.x;
