@@ -104,8 +104,8 @@ int hs_l1gs_get(hs_l1gs_t h, int what, double *out);
  * LDS ring of 8 / 16 chunks, 12 / 24 steps ahead (2 is the default).  Used by
  * the autotuner.  No counterpart in the reference (one kernel per sweep). */
 int hs_l1gs_set_variant(hs_l1gs_t h, int variant);
-/* Persisted state (no counterpart in the reference, which rebuilds; the
- * serving use in SURVEY's next steps): writes the built preconditioner --
+/* Persisted state (no counterpart in the reference, which rebuilds; for
+ * services that evict and re-admit systems): writes the built preconditioner --
  * partitioning, D_Dl1, packed triangles and (MatrixViewCache) the device
  * matrix, exactly as stored on the device -- to `path` with a checksum, and
  * recreates it on the device without the matrix or a rebuild.  Loading a
