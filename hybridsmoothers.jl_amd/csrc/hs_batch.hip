@@ -1,6 +1,6 @@
 // hs_batch.hip -- batched persistent PCG: many SPD systems of uneven size
-// solved together, each with an L1GS symmetric preconditioner, as a serving
-// engine (submit / step / collect).
+// solved together, each with an L1GS symmetric preconditioner or a 4th-kind
+// l1-Jacobi Chebyshev smoother, as a serving engine (submit / step / collect).
 //
 // No counterpart in the reference, whose users solve one system per call
 // (solve(LinearProblem(A, b), KrylovJL_CG(); Pl = P), README.md:27-29); every
@@ -10,7 +10,8 @@
 // Device state:
 //  * one arena of HBM, carved by a host first-fit allocator into one block
 //    per resident system: its sliced-ELL matrix, row lengths, right-hand side,
-//    packed triangles, D_Dl1, CG vectors and reduction partials;
+//    the smoother's arrays (packed triangles and D_Dl1, or l1inv, FP32 values
+//    and the Chebyshev z and r vectors), CG vectors and reduction partials;
 //  * a slot table (descriptor + CG scalars per resident system);
 //  * a tile table: one entry per (system, 256-row tile) of every active system.
 //
@@ -21,6 +22,16 @@
 //   k_b_sweep     tiles: r -= alpha q, x += alpha p, z = M^-1 r, r'z partial
 //                 (a newly admitted system: r = b, x = 0, z = M^-1 b)
 //   k_b_fin_gamma systems: the CG scalars and the stopping test
+// With the Chebyshev smoother of degree d the sweep is replaced by d launches
+// (chebyshev.jl:158-183 as a left preconditioner, x0 = 0: the A*0 of the
+// first step and the last step's SpMV are dead):
+//   k_bc_init     tiles: r -= alpha q, x += alpha p, z_1 = b_1 l1inv r
+//   k_bc_step     tiles, k = 1 .. d-1: r_k = r_{k-1} - A z_k, the next z,
+//                 z_cg = z_1 + ... + z_d; the last one also the r'z partial
+// and a system's rho comes from power-iteration steps (chebyshev.jl:40-54) run
+// by the same launches before its first CG step (phase PH_POWER: w = (A v) .*
+// l1inv and ||w||^2 in k_b_spmv, rho = ||w|| in k_b_fin_delta, v = w / rho in
+// k_bc_init, the step count in k_b_fin_gamma).
 // The sweep is persistent: a fixed grid, each workgroup walking a contiguous
 // run of the tile table (neighbouring tiles of a system share the workgroup's
 // XCD and its L2); the SpMV takes one tile per workgroup of a large grid.  A
@@ -52,7 +63,7 @@ constexpr int TILE = 256;         // rows per tile (= one 256-thread workgroup)
 // nearly empty engine would pay on every pass.
 constexpr int SPMV_GRIDS[3] = {1024, 8192, 65536};
 
-enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_RESID = 3, PH_DONE = 4 };
+enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_RESID = 3, PH_DONE = 4, PH_POWER = 5 };
 
 // Descriptor of a resident system (device array, one per slot): byte offsets
 // of its arrays in the arena.  The kernels add them to the arena base, a
@@ -61,6 +72,7 @@ enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_RESID = 3, PH_DONE = 4 };
 // counted against the LDS counter).
 struct Sys {
     int64_t sptr, col, val, T, D, x, r, z, pa, pb, q, b, part;
+    int64_t l1, l1s, v32, za, zb, rc;  // Chebyshev
     int64_t n, nparts;
     int32_t ntiles, pad;
 };
@@ -75,6 +87,12 @@ struct SysP {
     double *x, *r, *z, *pa, *pb, *q;
     const double *b;
     double *part;  // two partials per tile
+    // Chebyshev: l1inv in FP64 (power iteration) and in the smoother's type,
+    // the FP32 matrix values (FP32 smoother), the z ping-pong and r of the
+    // recurrence (smoother's type)
+    const double *l1;
+    const uint8_t *l1s, *v32;
+    uint8_t *za, *zb, *rc;
     int64_t n, nparts;
     int32_t ntiles;
 };
@@ -95,6 +113,12 @@ __device__ __forceinline__ SysP load_sys(const Sys *__restrict__ sys, int s, uin
     S.q = reinterpret_cast<double *>(arena + d.q);
     S.b = reinterpret_cast<const double *>(arena + d.b);
     S.part = reinterpret_cast<double *>(arena + d.part);
+    S.l1 = reinterpret_cast<const double *>(arena + d.l1);
+    S.l1s = arena + d.l1s;
+    S.v32 = arena + d.v32;
+    S.za = arena + d.za;
+    S.zb = arena + d.zb;
+    S.rc = arena + d.rc;
     S.n = d.n;
     S.nparts = d.nparts;
     S.ntiles = d.ntiles;
@@ -104,6 +128,9 @@ __device__ __forceinline__ SysP load_sys(const Sys *__restrict__ sys, int s, uin
 struct State {
     CGDev cg;
     int phase, pad;
+    double rho;      // Chebyshev: the power iteration's current estimate (1 before it)
+    int pw, pmax;    // power-iteration steps done and to do
+    int pstop, pad2; // rho == 0 ended the power iteration
 };
 
 struct Tile {
@@ -155,7 +182,7 @@ __global__ __launch_bounds__(256) void k_b_spmv(const Tile *__restrict__ tiles,
             pold = (k & 1) ? S.pa : S.pb;
             pnew = (k & 1) ? S.pb : S.pa;
         }
-        if (ph != PH_ITER && ph != PH_RESID) continue;  // uniform in the workgroup
+        if (ph != PH_ITER && ph != PH_RESID && ph != PH_POWER) continue;  // uniform in the workgroup
         const SellView<double> A{S.sptr, S.col, S.val, S.n};
         const int64_t i = (int64_t)tl.tile * TILE + threadIdx.x;
         if (ph == PH_ITER) {
@@ -163,6 +190,18 @@ __global__ __launch_bounds__(256) void k_b_spmv(const Tile *__restrict__ tiles,
             if (i < S.n) d = cg_spmv_row(A, S.z, pold, pnew, S.q, first, beta, i);
             d = block_sum(d, sm);
             if (threadIdx.x == 0) S.part[2 * tl.tile] = d;
+        } else if (ph == PH_POWER) {
+            // power-iteration step of a Chebyshev system: w = (A v) .* l1inv
+            // (v in pa, w to pb) and the tile's share of ||w||^2, as k_pow_spmv
+            double sq = 0.0;
+            if (i < S.n) {
+                double wi = row_dot(A, S.pa, i);
+                wi = wi * S.l1[i];
+                S.pb[i] = wi;
+                sq = wi * wi;
+            }
+            sq = block_sum(sq, sm);
+            if (threadIdx.x == 0) S.part[2 * tl.tile] = sq;
         } else {
             // true residual of a finished system, ||b - A x||^2 and ||b||^2
             double t2 = 0.0, b2 = 0.0;
@@ -195,6 +234,13 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_b_fin_delta(const int32_t *__rest
         if (ph == PH_ITER) {
             const double tot = finalize_sum(S.part, (unsigned)S.ntiles, 2, sm);
             if (threadIdx.x == 0 && !cg_scalars_delta(st[s].cg, tot)) st[s].phase = PH_RESID;
+        } else if (ph == PH_POWER) {  // rho = ||w|| (k_pow_fin)
+            const double tot = finalize_sum(S.part, (unsigned)S.ntiles, 2, sm);
+            if (threadIdx.x == 0) {
+                const double rho = sqrt(tot);
+                st[s].rho = rho;
+                if (rho == 0.0) st[s].pstop = 1;
+            }
         } else if (ph == PH_RESID) {
             const double e2 = finalize_sum(S.part, (unsigned)S.ntiles, 2, sm);
             const double b2 = finalize_sum(S.part + 1, (unsigned)S.ntiles, 2, sm);
@@ -290,6 +336,15 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_b_fin_gamma(const int32_t *__rest
     for (int a = blockIdx.x; a < na; a += gridDim.x) {
         const int s = act[a];
         const int ph = st[s].phase;
+        if (ph == PH_POWER) {  // one more power-iteration step done
+            __syncthreads();   // every thread has read the phase before it changes
+            if (threadIdx.x == 0) {
+                const int pw = st[s].pw + 1;
+                st[s].pw = pw;
+                if (st[s].pstop || pw >= st[s].pmax) st[s].phase = PH_INIT;
+            }
+            continue;
+        }
         if (ph != PH_INIT && ph != PH_ITER) continue;
         const SysP S = load_sys(sys, s, arena);
         const double tot = finalize_sum(S.part, (unsigned)S.ntiles, 2, sm);
@@ -297,6 +352,129 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_b_fin_gamma(const int32_t *__rest
             const bool done = (ph == PH_INIT) ? cg_scalars_start(st[s].cg, tot)
                                               : cg_scalars_next(st[s].cg, tot);
             st[s].phase = done ? PH_RESID : PH_ITER;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the Chebyshev smoother (4th kind, degree d >= 2) as the preconditioner
+// ---------------------------------------------------------------------------
+// Coefficients of step k, formed in FP64 from the system's rho and rounded to
+// the smoother's type, as the single-system plan forms them on the host:
+// a_k = (2k-3)/(2k+1), b_k = (8k-4)/((2k+1) rho)  (chebyshev.jl:158-183).
+template <class T>
+__device__ __forceinline__ void cheb_ab(int k, double rho, T &ak, T &bk) {
+    ak = (T)((double)(2 * k - 3) / (double)(2 * k + 1));
+    bk = (T)((double)(8 * k - 4) / ((double)(2 * k + 1) * rho));
+}
+
+template <class T>
+__device__ __forceinline__ SellView<T> cheb_view(const SysP &S) {
+    if constexpr (sizeof(T) == 8) return {S.sptr, S.col, S.val, S.n};
+    else return {S.sptr, S.col, reinterpret_cast<const float *>(S.v32), S.n};
+}
+
+// Pointwise start of the preconditioner, one tile per workgroup over the SpMV
+// grid.  Iteration: r -= alpha q, x += alpha p (the CG's own update, as in
+// cg_prologue), z_1 = a_1 0 + b_1 l1inv r (k_c4_init_cg); admission: r = b,
+// x = 0, z_1 from b (k_cg_start, then k_c4_init); power iteration: v = w / rho
+// (k_pow_scale).
+template <class T>
+__global__ __launch_bounds__(256) void k_bc_init(const Tile *__restrict__ tiles,
+                                                 const Counts *__restrict__ cnt,
+                                                 const Sys *__restrict__ sys,
+                                                 const State *__restrict__ st,
+                                                 uint8_t *__restrict__ arena) {
+    const int nt = cnt->ntiles;
+    int cur = -1, ph = PH_EMPTY;
+    SysP S{};
+    T a1 = (T)0, b1 = (T)0;
+    double alpha = 0.0, rho = 1.0;
+    const double *p = nullptr;
+    bool stop = false;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        if (tl.slot != cur) {
+            cur = tl.slot;
+            ph = st[cur].phase;
+            S = load_sys(sys, cur, arena);
+            rho = st[cur].rho;
+            stop = st[cur].pstop != 0;
+            cheb_ab<T>(1, rho, a1, b1);
+            alpha = st[cur].cg.gamma / st[cur].cg.delta;
+            p = (st[cur].cg.iter & 1) ? S.pb : S.pa;
+        }
+        const int64_t i = (int64_t)tl.tile * TILE + threadIdx.x;
+        if (i >= S.n) continue;
+        if (ph == PH_POWER) {
+            if (!stop) S.pa[i] = S.pb[i] / rho;
+            continue;
+        }
+        if (ph != PH_INIT && ph != PH_ITER) continue;
+        double ri;
+        if (ph == PH_INIT) {
+            ri = S.b[i];
+            S.x[i] = 0.0;
+        } else {
+            ri = S.r[i] - alpha * S.q[i];
+            S.x[i] = S.x[i] + alpha * p[i];
+        }
+        S.r[i] = ri;
+        const T rt = (T)ri;
+        const T *l1 = reinterpret_cast<const T *>(S.l1s);
+        reinterpret_cast<T *>(S.za)[i] = a1 * (T)0 + b1 * l1[i] * rt;
+    }
+}
+
+// Step k of d-1 (k_c4_step; the last one is k_c4_step_dot): r_k = r_{k-1} -
+// A z_k with r_0 the CG residual, z_cg = z_1 + ... (z_cg = 0 + z_1 at k = 1),
+// z_{k+1} = a_{k+1} z_k + b_{k+1} l1inv r_k; the last step adds z_d to z_cg
+// instead of storing it and forms the tile's r'z partial.  z_k is read from
+// za (k odd) or zb and z_{k+1} written to the other.
+template <class T>
+__global__ __launch_bounds__(256) void k_bc_step(const Tile *__restrict__ tiles,
+                                                 const Counts *__restrict__ cnt,
+                                                 const Sys *__restrict__ sys,
+                                                 const State *__restrict__ st,
+                                                 uint8_t *__restrict__ arena, int k, int d) {
+    __shared__ double sm[4];
+    const bool last = (k == d - 1);
+    const int nt = cnt->ntiles;
+    int cur = -1, ph = PH_EMPTY;
+    SysP S{};
+    T an = (T)0, bn = (T)0;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        if (tl.slot != cur) {
+            cur = tl.slot;
+            ph = st[cur].phase;
+            S = load_sys(sys, cur, arena);
+            cheb_ab<T>(k + 1, st[cur].rho, an, bn);
+        }
+        if (ph != PH_INIT && ph != PH_ITER) continue;  // uniform in the workgroup
+        const SellView<T> A = cheb_view<T>(S);
+        const T *l1 = reinterpret_cast<const T *>(S.l1s);
+        const T *zk = reinterpret_cast<const T *>((k & 1) ? S.za : S.zb);
+        T *znext = reinterpret_cast<T *>((k & 1) ? S.zb : S.za);
+        T *rc = reinterpret_cast<T *>(S.rc);
+        const int64_t i = (int64_t)tl.tile * TILE + threadIdx.x;
+        double v = 0.0;
+        if (i < S.n) {
+            const T rin = (k == 1) ? (T)S.r[i] : rc[i];
+            const T ri = row_sub(A, zk, i, rin);
+            if (!last) rc[i] = ri;
+            const T zi = zk[i];
+            double xi = (k == 1) ? 0.0 : S.z[i];
+            xi = xi + (double)zi;
+            const T zn = an * zi + bn * l1[i] * ri;
+            if (!last) znext[i] = zn;
+            else xi = xi + (double)zn;
+            S.z[i] = xi;
+            if (last) v = S.r[i] * xi;
+        }
+        if (last) {
+            v = block_sum(v, sm);
+            if (threadIdx.x == 0) S.part[2 * tl.tile] = v;
         }
     }
 }
@@ -340,6 +518,28 @@ __global__ void k_b_setup(int64_t n, int64_t P, const int64_t *__restrict__ sptr
     D[g] = (TC)dl1_rule(a, d, eta);
 }
 
+// Chebyshev setup of one system: l1inv_i = 1 / sum_j |a_ij| in row order
+// (k_l1inv: _l1_rowsum, chebyshev.jl:27-36) and its FP32 copy.
+__global__ void k_bc_setup(SellView<double> A, double *__restrict__ l1,
+                           float *__restrict__ l1s) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const double r = 1.0 / row_abs_sum(A, i);
+    l1[i] = r;
+    if (l1s) l1s[i] = (float)r;
+}
+
+// FP32 matrix values of the FP32 smoother (as CSR::ensure_fp32) and the
+// power iteration's start vector v0 = 1/sqrt(n)
+__global__ void k_bc_f32(int64_t m, const double *__restrict__ v, float *__restrict__ o) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) o[i] = (float)v[i];
+}
+__global__ void k_bc_fill(int64_t n, double *__restrict__ v, double a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = a;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -351,11 +551,23 @@ struct Layout {
     int64_t n = 0, nslices = 0, nent = 0, nparts = 0, ntiles = 0, tri = 0;
     int64_t o_sptr = 0, o_col = 0, o_val = 0, o_rlen = 0, o_b = 0, upload = 0;
     int64_t o_T = 0, o_D = 0, o_x = 0, o_r = 0, o_z = 0, o_pa = 0, o_pb = 0, o_q = 0, o_part = 0;
+    int64_t o_l1 = 0, o_l1s = 0, o_v32 = 0, o_za = 0, o_zb = 0, o_rc = 0;
     int64_t total = 0;
 };
 
-static Layout make_layout(int64_t n, const int64_t *ptr, int64_t P, int tsz, int dsz,
-                          bool one_tri) {
+// What a system's block holds besides the matrix and the CG vectors: the
+// L1GS arrays (partsize P, triangle and D element sizes, one or two
+// triangles) or the Chebyshev ones (element size tsz, degree).
+struct Shape {
+    int64_t P = 32;
+    int tsz = 8, dsz = 8;
+    bool one_tri = false, cheb = false;
+    int degree = 2;
+};
+
+static Layout make_layout(int64_t n, const int64_t *ptr, const Shape &sh) {
+    const int64_t P = sh.P;
+    const int tsz = sh.tsz, dsz = sh.dsz;
     Layout L;
     L.n = n;
     L.nslices = (n + 63) / 64;
@@ -365,10 +577,10 @@ static Layout make_layout(int64_t n, const int64_t *ptr, int64_t P, int tsz, int
             w = std::max<int64_t>(w, ptr[i + 1] - ptr[i]);
         L.nent += 64 * w;
     }
-    L.nparts = (n + P - 1) / P;
+    L.nparts = sh.cheb ? 0 : (n + P - 1) / P;
     L.ntiles = (n + TILE - 1) / TILE;
     L.tri = L.nparts * P * (P - 1);  // full blocks, the last one padded
-    if (one_tri) L.tri /= 2;
+    if (sh.one_tri) L.tri /= 2;
     L.o_sptr = 0;
     L.o_col = align256(L.o_sptr + (L.nslices + 1) * 8);
     L.o_val = align256(L.o_col + L.nent * 4);
@@ -377,7 +589,7 @@ static Layout make_layout(int64_t n, const int64_t *ptr, int64_t P, int tsz, int
     L.upload = L.o_b + n * 8;
     L.o_T = align256(L.upload);
     L.o_D = align256(L.o_T + L.tri * tsz);
-    L.o_x = align256(L.o_D + n * dsz);
+    L.o_x = align256(L.o_D + (sh.cheb ? 0 : n * dsz));
     L.o_r = align256(L.o_x + n * 8);
     L.o_z = align256(L.o_r + n * 8);
     L.o_pa = align256(L.o_z + n * 8);
@@ -385,6 +597,16 @@ static Layout make_layout(int64_t n, const int64_t *ptr, int64_t P, int tsz, int
     L.o_q = align256(L.o_pb + n * 8);
     L.o_part = align256(L.o_q + n * 8);
     L.total = align256(L.o_part + 2 * L.ntiles * 8);
+    if (sh.cheb) {
+        const bool f32 = tsz == 4;
+        L.o_l1 = L.total;
+        L.o_l1s = f32 ? align256(L.o_l1 + n * 8) : L.o_l1;  // FP64: the same array
+        L.o_v32 = align256(L.o_l1s + (f32 ? n * 4 : n * 8));
+        L.o_za = align256(L.o_v32 + (f32 ? L.nent * 4 : 0));
+        L.o_zb = align256(L.o_za + n * tsz);
+        L.o_rc = align256(L.o_zb + n * tsz);
+        L.total = align256(L.o_rc + (sh.degree >= 3 ? n * tsz : 0));
+    }
     return L;
 }
 
@@ -499,6 +721,7 @@ struct Engine {
     int tsz = 8, dsz = 8;
     int grid_spmv = 0, grid_sweep = 0;  // workgroups of the two tile kernels
     bool one_tri = false;               // packed lower triangle only (MODE_SYMT)
+    bool cheb = false;                  // Chebyshev smoother instead of L1GS
     uint8_t *arena_dev = nullptr;
     Arena arena;
     uint8_t *staging = nullptr;
@@ -532,6 +755,16 @@ struct Engine {
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    Shape shape() const {
+        Shape sh;
+        sh.P = cfg.partsize;
+        sh.tsz = tsz;
+        sh.dsz = dsz;
+        sh.one_tri = one_tri;
+        sh.cheb = cheb;
+        sh.degree = cfg.degree;
+        return sh;
     }
     ~Engine();
     void create(const hs_batch_config &c);
@@ -609,12 +842,17 @@ static const void *sweep_fn(int P, int prec, bool symt) {
 
 void Engine::create(const hs_batch_config &c) {
     cfg = c;
+    cheb = c.smoother == HS_PCG_CHEB;
     tsz = c.precision == HS_PREC_FP64 ? 8 : (c.precision == HS_PREC_FP32 ? 4 : 2);
     dsz = c.precision == HS_PREC_FP64 ? 8 : 4;
-    const bool fits = symt_fits_rt((int)c.partsize, c.precision);
-    HS_REQUIRE(c.triangles != 1 || fits, HS_EUNSUPPORTED,
-               "one-triangle storage needs partsize 16, or partsize 32 or 64 in FP32/BF16");
-    one_tri = c.triangles == 1 || (c.triangles == 0 && fits);
+    if (cheb) {
+        one_tri = false;
+    } else {
+        const bool fits = symt_fits_rt((int)c.partsize, c.precision);
+        HS_REQUIRE(c.triangles != 1 || fits, HS_EUNSUPPORTED,
+                   "one-triangle storage needs partsize 16, or partsize 32 or 64 in FP32/BF16");
+        one_tri = c.triangles == 1 || (c.triangles == 0 && fits);
+    }
     int dev = 0;
     HS_CHECK_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -624,8 +862,10 @@ void Engine::create(const hs_batch_config &c) {
     // round of workgroups would wait for the first to finish).
     const int cus = prop.multiProcessorCount;
     grid_spmv = SPMV_GRIDS[2];
-    grid_sweep = c.grid > 0 ? c.grid
-                            : resident_grid(sweep_fn((int)c.partsize, c.precision, one_tri), cus);
+    grid_sweep = cheb ? 0
+                      : (c.grid > 0 ? c.grid
+                                    : resident_grid(sweep_fn((int)c.partsize, c.precision, one_tri),
+                                                    cus));
     HS_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HS_CHECK_HIP(hipStreamCreateWithFlags(&us, hipStreamNonBlocking));
     HS_CHECK_HIP(hipEventCreateWithFlags(&ev_step, hipEventDisableTiming));
@@ -698,7 +938,20 @@ void Engine::enqueue_pass(hipStream_t s, unsigned spmv_grid) {
     hipLaunchKernelGGL(k_b_spmv, dim3(spmv_grid), dim3(TILE), 0, s, tl, cn, sy, stt, arena_dev);
     hipLaunchKernelGGL(k_b_fin_delta, dim3(fin_grid), dim3(FIN_BLOCK), 0, s, d_act.get(), cn, sy,
                        stt, arena_dev);
-    if (cfg.partsize == 16)
+    if (cheb) {
+        const dim3 g(spmv_grid), t(TILE);
+        if (cfg.precision == HS_PREC_FP64) {
+            hipLaunchKernelGGL(k_bc_init<double>, g, t, 0, s, tl, cn, sy, stt, arena_dev);
+            for (int k = 1; k < cfg.degree; ++k)
+                hipLaunchKernelGGL(k_bc_step<double>, g, t, 0, s, tl, cn, sy, stt, arena_dev, k,
+                                   cfg.degree);
+        } else {
+            hipLaunchKernelGGL(k_bc_init<float>, g, t, 0, s, tl, cn, sy, stt, arena_dev);
+            for (int k = 1; k < cfg.degree; ++k)
+                hipLaunchKernelGGL(k_bc_step<float>, g, t, 0, s, tl, cn, sy, stt, arena_dev, k,
+                                   cfg.degree);
+        }
+    } else if (cfg.partsize == 16)
         launch_sweep<16>(cfg.precision, one_tri, grid_sweep, s, tl, cn, sy, stt, arena_dev);
     else if (cfg.partsize == 32)
         launch_sweep<32>(cfg.precision, one_tri, grid_sweep, s, tl, cn, sy, stt, arena_dev);
@@ -727,7 +980,7 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
     HS_REQUIRE(ptr[0] == 1, HS_EINVAL, "pointer array must be 1-based (ptr[0] == 1)");
     for (int64_t i = 0; i < n; ++i)
         HS_REQUIRE(ptr[i + 1] >= ptr[i], HS_EINVAL, "pointer array must be non-decreasing");
-    const Layout L = make_layout(n, ptr, cfg.partsize, tsz, dsz, one_tri);
+    const Layout L = make_layout(n, ptr, shape());
     HS_REQUIRE(L.upload <= cfg.staging_bytes, HS_EINVAL,
                "system larger than the engine's upload staging buffer");
     HS_REQUIRE(L.total <= cfg.arena_bytes, HS_EINVAL, "system larger than the engine's arena");
@@ -752,7 +1005,20 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
         const auto *cl = reinterpret_cast<const int32_t *>(blk + L.o_col);
         const auto *vl = reinterpret_cast<const double *>(blk + L.o_val);
         const auto *rl = reinterpret_cast<const int32_t *>(blk + L.o_rlen);
-        if (cfg.precision == HS_PREC_FP64)
+        if (cheb) {
+            const bool f32 = cfg.precision == HS_PREC_FP32;
+            hipLaunchKernelGGL(k_bc_setup, g, t, 0, us, SellView<double>{sp, cl, vl, n},
+                               reinterpret_cast<double *>(blk + L.o_l1),
+                               f32 ? reinterpret_cast<float *>(blk + L.o_l1s) : nullptr);
+            if (f32)
+                hipLaunchKernelGGL(k_bc_f32, dim3(grid_for(L.nent, 256)), t, 0, us, L.nent, vl,
+                                   reinterpret_cast<float *>(blk + L.o_v32));
+            // the power iteration's v0 (in pa), as hs_cheb_build forms it
+            if (cfg.power_iters > 0)
+                hipLaunchKernelGGL(k_bc_fill, g, t, 0, us, n,
+                                   reinterpret_cast<double *>(blk + L.o_pa),
+                                   1.0 / std::sqrt((double)n));
+        } else if (cfg.precision == HS_PREC_FP64)
             hipLaunchKernelGGL((k_b_setup<double, double>), g, t, 0, us, n, cfg.partsize, sp, cl,
                                vl, rl, cfg.eta, reinterpret_cast<double *>(blk + L.o_D),
                                reinterpret_cast<double *>(blk + L.o_T), mode);
@@ -847,6 +1113,12 @@ void Engine::step() {
             d.q = o + L.o_q;
             d.b = o + L.o_b;
             d.part = o + L.o_part;
+            d.l1 = o + L.o_l1;
+            d.l1s = o + L.o_l1s;
+            d.v32 = o + L.o_v32;
+            d.za = o + L.o_za;
+            d.zb = o + L.o_zb;
+            d.rc = o + L.o_rc;
             d.n = L.n;
             d.nparts = L.nparts;
             d.ntiles = (int32_t)L.ntiles;
@@ -855,7 +1127,9 @@ void Engine::step() {
             h.cg.atol = S.atol;
             h.cg.rtol = S.rtol;
             h.cg.itmax = S.itmax;
-            h.phase = PH_INIT;
+            h.rho = 1.0;  // hs_cheb_build's estimate before (or without) the power iteration
+            h.pmax = cheb ? cfg.power_iters : 0;
+            h.phase = h.pmax > 0 ? PH_POWER : PH_INIT;
             HS_CHECK_HIP(hipStreamWaitEvent(cs, S.ready, 0));
             HS_CHECK_HIP(hipMemcpyAsync(d_sys.get() + s, &h_sys[s], sizeof(Sys),
                                         hipMemcpyHostToDevice, cs));
@@ -938,11 +1212,21 @@ int hs_batch_create(const hs_batch_config *cfg, hs_batch_t *out) {
         HS_REQUIRE(cfg->max_systems > 0 && cfg->max_systems <= (1 << 20), HS_EINVAL,
                    "max_systems must be in [1, 2^20]");
         HS_REQUIRE(cfg->staging_bytes > 0, HS_EINVAL, "staging_bytes must be positive");
-        HS_REQUIRE(cfg->partsize == 16 || cfg->partsize == 32 || cfg->partsize == 64,
+        HS_REQUIRE(cfg->smoother == 0 || cfg->smoother == HS_PCG_L1GS ||
+                       cfg->smoother == HS_PCG_CHEB,
+                   HS_EINVAL, "smoother must be 0 or HS_PCG_L1GS (L1GS) or HS_PCG_CHEB");
+        const bool cheb = cfg->smoother == HS_PCG_CHEB;
+        HS_REQUIRE(cheb || cfg->partsize == 16 || cfg->partsize == 32 || cfg->partsize == 64,
                    HS_EUNSUPPORTED, "the batched engine takes partsize 16, 32 or 64");
         HS_REQUIRE(cfg->precision == HS_PREC_FP64 || cfg->precision == HS_PREC_FP32 ||
                        cfg->precision == HS_PREC_BF16,
                    HS_EINVAL, "unknown precision");
+        HS_REQUIRE(!cheb || cfg->precision != HS_PREC_BF16, HS_EUNSUPPORTED,
+                   "Chebyshev smoothers run in FP64 or FP32");
+        HS_REQUIRE(!cheb || (cfg->degree >= 2 && cfg->degree <= 16), HS_EUNSUPPORTED,
+                   "the batched Chebyshev smoother takes degree 2 to 16");
+        HS_REQUIRE(!cheb || cfg->power_iters >= 0, HS_EINVAL,
+                   "power_iters must be non-negative");
         HS_REQUIRE(cfg->passes >= 1 && cfg->passes <= 1024, HS_EINVAL,
                    "passes must be in [1, 1024]");
         HS_REQUIRE(cfg->grid >= 0, HS_EINVAL, "grid must be >= 0");
@@ -958,7 +1242,7 @@ int hs_batch_footprint(hs_batch_t e, int64_t n, const int64_t *ptr, int64_t *byt
     return guard([&] {
         HS_REQUIRE(e && ptr && bytes, HS_EINVAL, "NULL argument");
         HS_REQUIRE(n > 0, HS_EINVAL, "system size must be positive");
-        *bytes = batch::make_layout(n, ptr, e->cfg.partsize, e->tsz, e->dsz, e->one_tri).total;
+        *bytes = batch::make_layout(n, ptr, e->shape()).total;
     });
 }
 
@@ -1013,14 +1297,16 @@ int hs_batch_info(hs_batch_t e, int64_t *out) {
         out[8] = (int64_t)e->results.size();
         out[9] = e->grid_spmv;
         out[10] = e->grid_sweep;
-        out[11] = e->one_tri ? 1 : 2;  // packed triangles per partition
+        out[11] = e->cheb ? 0 : (e->one_tri ? 1 : 2);  // packed triangles per partition
     });
 }
 
 int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out) {
     return guard([&] {
         HS_REQUIRE(e && out, HS_EINVAL, "NULL argument");
-        HS_REQUIRE(what >= 0 && what <= 6, HS_EINVAL, "unknown quantity");
+        HS_REQUIRE(what >= 0 && what <= 8, HS_EINVAL, "unknown quantity");
+        HS_REQUIRE((what != 5 || !e->cheb) && (what < 7 || e->cheb), HS_EINVAL,
+                   "quantity of the other smoother");
         e->wait_step();
         HS_CHECK_HIP(hipStreamSynchronize(e->us));
         const batch::Slot *S = nullptr;
@@ -1040,8 +1326,15 @@ int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out) {
                 HS_CHECK_HIP(hipMemcpy(f.data(), blk + L.o_D, L.n * 4, hipMemcpyDeviceToHost));
                 for (int64_t i = 0; i < L.n; ++i) out[i] = f[i];
             }
-        } else {  // the two partials of every tile
+        } else if (what == 6) {  // the two partials of every tile
             HS_CHECK_HIP(hipMemcpy(out, blk + L.o_part, 2 * L.ntiles * 8, hipMemcpyDeviceToHost));
+        } else if (what == 7) {  // l1inv (FP64)
+            HS_CHECK_HIP(hipMemcpy(out, blk + L.o_l1, L.n * 8, hipMemcpyDeviceToHost));
+        } else {  // rho
+            batch::State h;
+            const int64_t slot = S - e->slots.data();
+            HS_CHECK_HIP(hipMemcpy(&h, e->d_st.get() + slot, sizeof h, hipMemcpyDeviceToHost));
+            out[0] = h.rho;
         }
     });
 }

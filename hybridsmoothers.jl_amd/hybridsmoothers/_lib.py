@@ -64,7 +64,8 @@ class BatchConfig(C.Structure):
     _fields_ = [("arena_bytes", C.c_int64), ("max_systems", C.c_int64),
                 ("staging_bytes", C.c_int64), ("partsize", C.c_int64),
                 ("precision", C.c_int), ("passes", C.c_int), ("grid", C.c_int),
-                ("eta", C.c_double), ("triangles", C.c_int)]
+                ("eta", C.c_double), ("triangles", C.c_int), ("smoother", C.c_int),
+                ("degree", C.c_int), ("power_iters", C.c_int)]
 
 
 class BatchResult(C.Structure):

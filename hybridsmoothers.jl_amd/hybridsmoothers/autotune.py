@@ -73,11 +73,12 @@ def sample_systems(specs, rhs_seed: int = 1000):
 
 
 def time_batch(systems, partsize, passes, grid=0, precision="fp32", rtol=1e-8,
-               arena_bytes=None, max_systems=1024, reps=1):
+               arena_bytes=None, max_systems=1024, reps=1, **smoother):
     """Median wall time (s) of solving `systems` with one engine configuration,
-    after one warm-up solve; raises if any solve misses its tolerance."""
+    after one warm-up solve; raises if any solve misses its tolerance.
+    `smoother`: BatchSolver's smoother, degree and power_iters."""
     with BatchSolver(partsize=partsize, precision=precision, passes=passes, grid=grid,
-                     arena_bytes=arena_bytes, max_systems=max_systems) as eng:
+                     arena_bytes=arena_bytes, max_systems=max_systems, **smoother) as eng:
         eng.solve_all(systems[: max(1, len(systems) // 8)], rtol=rtol)  # warm-up
         times, sols = [], None
         for _ in range(reps):
@@ -92,17 +93,20 @@ def time_batch(systems, partsize, passes, grid=0, precision="fp32", rtol=1e-8,
 
 
 def tune_batch(systems, partsizes=(16, 32, 64), passes=(4, 8, 16), grids=(0,),
-               precision="fp32", rtol=1e-8, arena_bytes=None, cache_path=None, log=None):
+               precision="fp32", rtol=1e-8, arena_bytes=None, cache_path=None, log=None,
+               **smoother):
     """Best (partsize, passes, grid) for `systems` by solves/s; returns
-    {"partsize", "passes", "grid", "solves_per_s", "table": [...]}."""
+    {"partsize", "passes", "grid", "solves_per_s", "table": [...]}.  With
+    smoother="cheb" partsize and grid do not apply: pass one of each."""
     sig = f"{len(systems)}:{sum(A[0].size - 1 for A, _ in systems)}"
-    key = f"batch|{_device_name()}|{precision}|{partsizes}|{passes}|{grids}|{sig}"
+    key = (f"batch|{_device_name()}|{precision}|{partsizes}|{passes}|{grids}|"
+           f"{sorted(smoother.items())}|{sig}")
     hit = _load(cache_path, key)
     if hit:
         return hit
     table = []
     for p, k, g in itertools.product(partsizes, passes, grids):
-        t, it = time_batch(systems, p, k, g, precision, rtol, arena_bytes)
+        t, it = time_batch(systems, p, k, g, precision, rtol, arena_bytes, **smoother)
         row = {"partsize": p, "passes": k, "grid": g, "seconds": round(t, 5),
                "solves_per_s": round(len(systems) / t, 2), "iters_mean": round(it, 1)}
         table.append(row)

@@ -6,10 +6,14 @@ Each system submitted here is solved exactly as
 
     P = L1GSPrecBuilder(ROCBackend(), ...)(A, partsize, isSymA=True, sweep=SymmetricSweep(),
                                            precision=precision)
+    # or, smoother="cheb":
+    P = _build_smoother(ChebyshevFourth(degree=degree, precision=precision), A)
     pcg_solve(DeviceMatrix(A), b, Pl=P, reltol=rtol, abstol=atol)
 
 would solve it (FP64 CG, Krylov.jl's stopping rule, x0 = 0), but many systems
-share every kernel launch.  The engine keeps matrices, preconditioners and CG
+share every kernel launch.  A Chebyshev system's rho comes from `power_iters`
+power-iteration steps that the engine runs for it, inside its passes, before
+its first CG iteration.  The engine keeps matrices, preconditioners and CG
 state resident in one device arena; `submit` returns None when the arena or
 the slot table is full (admission by device-memory footprint).
 
@@ -32,6 +36,7 @@ import scipy.sparse as sp
 from . import _lib as L
 
 _PREC = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32, "bf16": L.PREC_BF16}
+_SMOOTHER = {"l1gs": L.PCG_L1GS, "cheb": L.PCG_CHEB}
 _STATUS = {0: "solved", 1: "maxiters", 2: "breakdown"}
 
 
@@ -70,25 +75,40 @@ def csr_arrays(A):
             np.ascontiguousarray(A.data, dtype=np.float64))
 
 
-def iteration_bytes(ptr, partsize: int, precision: str, triangles: int = 2) -> int:
+def iteration_bytes(ptr, partsize: int, precision: str, triangles: int = 2,
+                    smoother: str = "l1gs", degree: int = 2) -> int:
     """Algorithmic HBM bytes of one fused CG iteration of one system in the
     engine: the sliced-ELL matrix (8-byte value + 4-byte column per stored
     entry, slice offsets), z and p_{k-1} gathered once and p_k, q written
-    (32 B/row) in the SpMV; the packed triangle(s) as stored (`triangles` =
-    1: L only, U = L^T), D_Dl1, r and x read and written, q and p read, z
-    written (56 B/row) in the sweep."""
+    (32 B/row) in the SpMV; with L1GS, the packed triangle(s) as stored
+    (`triangles` = 1: L only, U = L^T), D_Dl1, r and x read and written, q
+    and p read, z written (56 B/row) in the sweep; with the Chebyshev smoother
+    of degree d (element size e), the start (r, q, x, p, l1inv read; r, x,
+    z_1 written: 48 + 2e B/row) and d-1 steps, each reading the matrix in the
+    smoother's type, z_k, r_{k-1}, l1inv and z_cg and writing z_cg, r_k and
+    z_{k+1} (the last: reading r instead of writing r_k and z_{k+1})."""
     ptr = np.asarray(ptr, dtype=np.int64)
     n = ptr.size - 1
     lens = np.diff(ptr)
     pad = (-n) % 64
     w = np.concatenate([lens, np.zeros(pad, np.int64)]).reshape(-1, 64).max(axis=1)
     nent = int(64 * w.sum())
+    spmv = nent * 12 + (w.size + 1) * 8 + 32 * n
+    if smoother == "cheb":
+        e = 8 if precision == "fp64" else 4
+        tot = spmv + n * (48 + 2 * e)
+        for k in range(1, degree):
+            last = k == degree - 1
+            row = e + (8 if k == 1 else e) + e + (0 if k == 1 else 8) + 8
+            row += 8 if last else 2 * e
+            tot += nent * (e + 4) + (w.size + 1) * 8 + n * row
+        return tot
     tsz = {"fp64": 8, "fp32": 4, "bf16": 2}[precision]
     dsz = 8 if precision == "fp64" else 4
     P = partsize
     nparts = (n + P - 1) // P
     tri = nparts * P * (P - 1) * triangles // 2  # the last block is stored full size
-    return nent * 12 + (w.size + 1) * 8 + tri * tsz + n * (dsz + 88)
+    return spmv + tri * tsz + n * (dsz + 56)
 
 
 def default_arena_bytes(fraction: float = 0.5, cap: int = 96 << 30) -> int:
@@ -103,17 +123,22 @@ def default_arena_bytes(fraction: float = 0.5, cap: int = 96 << 30) -> int:
 class BatchSolver:
     def __init__(self, partsize: int = 32, precision: str = "fp32", arena_bytes: int | None = None,
                  max_systems: int = 1024, staging_bytes: int = 256 << 20, passes: int = 8,
-                 grid: int = 0, eta: float = 1.5, triangles: int = 0, device: int = 0):
+                 grid: int = 0, eta: float = 1.5, triangles: int = 0, device: int = 0,
+                 smoother: str = "l1gs", degree: int = 2, power_iters: int = 50):
         L.init(device)
         if precision not in _PREC:
             raise L.HSError(L.HS_EINVAL, f"unknown precision {precision!r}")
+        if smoother not in _SMOOTHER:
+            raise L.HSError(L.HS_EINVAL, f"unknown smoother {smoother!r}")
         cfg = L.BatchConfig(arena_bytes=int(arena_bytes or default_arena_bytes()),
                             max_systems=int(max_systems), staging_bytes=int(staging_bytes),
                             partsize=int(partsize), precision=_PREC[precision],
                             passes=int(passes), grid=int(grid), eta=float(eta),
-                            triangles=int(triangles))
+                            triangles=int(triangles), smoother=_SMOOTHER[smoother],
+                            degree=int(degree), power_iters=int(power_iters))
         self.config = cfg
         self.partsize, self.precision, self.passes = int(partsize), precision, int(passes)
+        self.smoother, self.degree = smoother, int(degree)
         self._h = C.c_void_p()
         L.check(L.lib.hs_batch_create(C.byref(cfg), C.byref(self._h)))
         self.triangles = self.info()["triangles"]
@@ -183,13 +208,14 @@ class BatchSolver:
                 "grid_sweep", "triangles")
         return dict(zip(keys, (int(t) for t in v)))
 
-    _PEEK = {"x": 0, "r": 1, "z": 2, "b": 3, "q": 4, "D": 5, "partials": 6}
+    _PEEK = {"x": 0, "r": 1, "z": 2, "b": 3, "q": 4, "D": 5, "partials": 6, "l1inv": 7,
+             "rho": 8}
 
     def peek(self, sid: int, what: str) -> np.ndarray:
         """A device vector of a resident system after the last step (diagnostics)."""
         x, _ = self._live[sid]
         n = x.size
-        size = 2 * ((n + 255) // 256) if what == "partials" else n
+        size = {"partials": 2 * ((n + 255) // 256), "rho": 1}.get(what, n)
         out = np.empty(size, np.float64)
         L.check(L.lib.hs_batch_peek(self._h, sid, self._PEEK[what], out.ctypes.data))
         return out

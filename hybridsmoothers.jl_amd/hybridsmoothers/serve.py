@@ -198,7 +198,8 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
                 pending.pop(k)
                 rows += b.size
                 ibytes[i] = iteration_bytes(A[0] if isinstance(A, tuple) else A.indptr + 1,
-                                            engine.partsize, engine.precision, engine.triangles)
+                                            engine.partsize, engine.precision, engine.triangles,
+                                            engine.smoother, engine.degree)
             if busy:
                 for s in engine.collect():
                     i = s.tag

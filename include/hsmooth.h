@@ -198,9 +198,12 @@ int hs_problem_lognormal(int64_t nx, int64_t ny, int64_t nz, double sigma, uint6
  * Many SPD systems of uneven size stay resident on the device and advance
  * together: every graph launch runs `passes` CG iterations of every active
  * system, each preconditioned by its own symmetric L1GS (isSymA, eta, the
- * engine's partsize and precision).  Each system's arithmetic is that of
- * hs_pcg_solve with that preconditioner (x0 = 0, FP64 CG, Krylov.jl's rule),
- * so its result does not depend on the other systems in the batch.
+ * engine's partsize and precision) or its own l1-Jacobi Chebyshev smoother of
+ * the 4th kind (the engine's degree and precision; rho from `power_iters`
+ * power-iteration steps, run inside the engine's passes before the system's
+ * first CG step).  Each system's arithmetic is that of hs_pcg_solve with that
+ * preconditioner (x0 = 0, FP64 CG, Krylov.jl's rule), so its result does not
+ * depend on the other systems in the batch.
  */
 typedef struct hs_batch *hs_batch_t;
 
@@ -217,6 +220,12 @@ typedef struct {
     int triangles;          /* packed triangles stored per partition: 1 the lower only (U = L^T,
                                half the bytes; partsize 16, 32 and 64 in FP32/BF16),
                                2 both, 0 automatic (1 where possible) */
+    int smoother;           /* 0 or HS_PCG_L1GS: L1GS (the fields above); HS_PCG_CHEB:
+                               ChebyshevFourth(degree) as built by hs_cheb_build (kind 4,
+                               precision FP64 or FP32; partsize, grid, eta and triangles
+                               are not used) */
+    int degree;             /* Chebyshev degree, 2 to 16 */
+    int power_iters;        /* Chebyshev power-iteration steps for rho (the reference's 50) */
 } hs_batch_config;
 
 typedef struct {
@@ -257,8 +266,9 @@ int hs_batch_collect(hs_batch_t e, hs_batch_result *out, int64_t max, int64_t *c
 int hs_batch_info(hs_batch_t e, int64_t *out);
 /* Copies a vector of a resident (submitted, not yet collected) system to host
  * `out` after the last step: what = 0 x, 1 r, 2 z, 3 b, 4 q (n doubles each),
- * 5 D_Dl1 (n, widened to FP64), 6 the two partials of each 256-row tile.
- * For tests and diagnostics. */
+ * 5 D_Dl1 (n, widened to FP64; L1GS), 6 the two partials of each 256-row
+ * tile, 7 l1inv (n, FP64; Chebyshev), 8 rho (one value; Chebyshev, final once
+ * the power iteration has ended).  For tests and diagnostics. */
 int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out);
 /* 1 while the last step is still running on the device, 0 once it has
  * finished (or none is pending), negative on a device error.  Lets a caller

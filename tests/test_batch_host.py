@@ -101,11 +101,16 @@ def test_batch_create_argument_errors():
     for change, code in ((dict(partsize=20), L.HS_EUNSUPPORTED), (dict(passes=0), L.HS_EINVAL),
                          (dict(arena_bytes=0), L.HS_EINVAL), (dict(max_systems=0), L.HS_EINVAL),
                          (dict(precision=7), L.HS_EINVAL), (dict(grid=-1), L.HS_EINVAL),
-                         (dict(triangles=3), L.HS_EINVAL)):
+                         (dict(triangles=3), L.HS_EINVAL), (dict(smoother=3), L.HS_EINVAL),
+                         (dict(smoother=L.PCG_CHEB, degree=1), L.HS_EUNSUPPORTED),
+                         (dict(smoother=L.PCG_CHEB, degree=17), L.HS_EUNSUPPORTED),
+                         (dict(smoother=L.PCG_CHEB, degree=2, precision=L.PREC_BF16),
+                          L.HS_EUNSUPPORTED),
+                         (dict(smoother=L.PCG_CHEB, degree=2, power_iters=-1), L.HS_EINVAL)):
         cfg = L.BatchConfig(**{**base, **change})
         assert L.lib.hs_batch_create(C.byref(cfg), C.byref(h)) == code, change
     assert L.lib.hs_batch_create(None, C.byref(h)) == L.HS_EINVAL
     assert L.lib.hs_batch_step(None) == L.HS_EINVAL
     cnt = C.c_int64(0)
     assert L.lib.hs_batch_collect(None, None, 0, C.byref(cnt)) == L.HS_EINVAL
-    assert C.sizeof(L.BatchConfig) == 64 and C.sizeof(L.BatchResult) == 80
+    assert C.sizeof(L.BatchConfig) == 72 and C.sizeof(L.BatchResult) == 80
