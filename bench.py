@@ -14,7 +14,7 @@ smoother and FP32 smoother inside FP64 CG, rtol 1e-8) and the CPU oracle
 baseline timed on this host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-serve] [--no-cpu]
-                    [--serve-only] [--serve-count 4096]
+                    [--serve-only] [--no-serve-cheb] [--serve-count 4096]
 """
 from __future__ import annotations
 
@@ -146,39 +146,28 @@ def cpu_baseline(A, partsize, budget_s=12.0):
     return out
 
 
-def serving_bench(args, log_fn=log):
-    """Batched persistent PCG serving the stream of SPD systems: N log-uniform in
-    [1e3, 2.6e5] rows, alternating 2D 5-point / 3D 7-point lognormal (sigma 1)
-    grids, FP32 L1GS partsize 32 inside FP64 CG to rtol 1e-8.  The autotuner
-    picks steps and grid on a separate sample; a saturation run (all arrivals
-    at t = 0) measures capacity, then a Poisson stream at 0.8 x capacity gives
-    latency percentiles."""
+def _serve_engine(args, specs, reqs, log_fn, label, engine_kw, tune_kw):
+    """Autotune (on a separate 48-system sample), then a saturation run (all
+    arrivals at t = 0: capacity) and a Poisson stream at 0.8 x capacity
+    (latency percentiles) through one engine configuration."""
     from hybridsmoothers import autotune, problems as pb, serve
     from hybridsmoothers.batch import BatchSolver
-    N = args.serve_count
-    specs = pb.serving_stream(N, seed=5)
-    log_fn(f"[bench] serving: autotune on a 48-system sample")
+    N = len(specs)
+    log_fn(f"[bench] serving {label}: autotune on a 48-system sample")
     sample = autotune.sample_systems(pb.serving_stream(48, seed=77))
-    tune = autotune.tune_batch(sample, partsizes=(32,), passes=(8, 16, 32), grids=(0, 2048),
-                               precision="fp32", arena_bytes=32 << 30, log=log_fn)
-    out = {"workload": f"{N} SPD systems, N log-uniform in [1e3, 2.6e5] (2D 5-point / 3D "
-                       f"7-point lognormal sigma=1, alternating), b = U[-1,1]; FP64 CG, FP32 "
-                       f"L1GS partsize 32 symmetric, rtol 1e-8",
-           "autotune": tune}
-    log_fn(f"[bench] serving: generating the {N} requests (clients' matrices, before the clock)")
-    t0 = time.perf_counter()
-    reqs = serve.Prepared(serve.generate(specs, workers=16))
-    out["generation_s"] = round(time.perf_counter() - t0, 3)
-    with BatchSolver(partsize=32, precision="fp32", passes=tune["passes"], grid=tune["grid"],
-                     max_systems=args.serve_slots) as eng:
-        log_fn(f"[bench] serving: saturation run ({N} systems at t=0)")
+    tune = autotune.tune_batch(sample, passes=(8, 16, 32), precision="fp32",
+                               arena_bytes=32 << 30, log=log_fn, **tune_kw, **engine_kw)
+    out = {"autotune": tune}
+    with BatchSolver(partsize=tune["partsize"], precision="fp32", passes=tune["passes"],
+                     grid=tune["grid"], max_systems=args.serve_slots, **engine_kw) as eng:
+        log_fn(f"[bench] serving {label}: saturation run ({N} systems at t=0)")
         rep_sat, _ = serve.serve(eng, specs, serve.poisson_arrivals(N, float("inf")),
                                  producer=reqs)
-        log_fn(f"[bench] serving: saturation {rep_sat.as_dict()}")
+        log_fn(f"[bench] serving {label}: saturation {rep_sat.as_dict()}")
         lam = 0.8 * rep_sat.solves_per_s
-        log_fn(f"[bench] serving: Poisson arrivals at {lam:.1f}/s")
+        log_fn(f"[bench] serving {label}: Poisson arrivals at {lam:.1f}/s")
         rep, _ = serve.serve(eng, specs, serve.poisson_arrivals(N, lam, seed=7), producer=reqs)
-        log_fn(f"[bench] serving: poisson {rep.as_dict()}")
+        log_fn(f"[bench] serving {label}: poisson {rep.as_dict()}")
         out["arena_bytes"] = eng.info()["arena_bytes"]
     out["saturation"] = rep_sat.as_dict()
     out["poisson"] = rep.as_dict()
@@ -186,6 +175,36 @@ def serving_bench(args, log_fn=log):
     out["latency_p50_ms"] = round(rep.latency_p50_ms, 3)
     out["latency_p99_ms"] = round(rep.latency_p99_ms, 3)
     out["capacity_solves_per_s"] = round(rep_sat.solves_per_s, 2)
+    return out
+
+
+def serving_bench(args, log_fn=log):
+    """Batched persistent PCG serving the stream of SPD systems: N log-uniform in
+    [1e3, 2.6e5] rows, alternating 2D 5-point / 3D 7-point lognormal (sigma 1)
+    grids, FP64 CG to rtol 1e-8 preconditioned by FP32 L1GS partsize 32 (the
+    top-level figures) and, as the stream's alternative smoother, by an FP32
+    Chebyshev 4th-kind smoother of degree 2 (`chebyshev_d2`, rho from 50
+    power-iteration steps in the engine).  Same requests for both."""
+    from hybridsmoothers import problems as pb, serve
+    N = args.serve_count
+    specs = pb.serving_stream(N, seed=5)
+    log_fn(f"[bench] serving: generating the {N} requests (clients' matrices, before the clock)")
+    t0 = time.perf_counter()
+    reqs = serve.Prepared(serve.generate(specs, workers=16))
+    gen_s = round(time.perf_counter() - t0, 3)
+    out = {"workload": f"{N} SPD systems, N log-uniform in [1e3, 2.6e5] (2D 5-point / 3D "
+                       f"7-point lognormal sigma=1, alternating), b = U[-1,1]; FP64 CG, FP32 "
+                       f"L1GS partsize 32 symmetric, rtol 1e-8"}
+    out.update(_serve_engine(args, specs, reqs, log_fn, "L1GS P=32", {},
+                             {"partsizes": (32,), "grids": (0, 2048)}))
+    out["generation_s"] = gen_s
+    if not args.no_serve_cheb:
+        ch = _serve_engine(args, specs, reqs, log_fn, "Chebyshev d=2",
+                           {"smoother": "cheb", "degree": 2, "power_iters": 50},
+                           {"partsizes": (32,), "grids": (0,)})
+        ch["workload"] = ("same stream and requests; FP64 CG, FP32 Chebyshev 4th kind degree 2 "
+                          "(l1-Jacobi, rho from 50 power-iteration steps), rtol 1e-8")
+        out["chebyshev_d2"] = ch
     return out
 
 
@@ -320,6 +339,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=1024)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true", help="only the serving benchmark")
+    ap.add_argument("--no-serve-cheb", action="store_true",
+                    help="serve the stream with the L1GS engine only")
     ap.add_argument("--batch-steady", choices=("2d", "3d"), default=None,
                     help="only the batched engine's steady state (for kernel profiles)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 192^3 mixed-precision PCG")
