@@ -22,16 +22,18 @@
 //   k_b_sweep     tiles: r -= alpha q, x += alpha p, z = M^-1 r, r'z partial
 //                 (a newly admitted system: r = b, x = 0, z = M^-1 b)
 //   k_b_fin_gamma systems: the CG scalars and the stopping test
-// With the Chebyshev smoother of degree d the sweep is replaced by d launches
-// (chebyshev.jl:158-183 as a left preconditioner, x0 = 0: the A*0 of the
-// first step and the last step's SpMV are dead):
-//   k_bc_init     tiles: r -= alpha q, x += alpha p, z_1 = b_1 l1inv r
-//   k_bc_step     tiles, k = 1 .. d-1: r_k = r_{k-1} - A z_k, the next z,
-//                 z_cg = z_1 + ... + z_d; the last one also the r'z partial
+// With the Chebyshev smoother of degree d the sweep is replaced by d - 1
+// launches (chebyshev.jl:158-183 as a left preconditioner, x0 = 0: the A*0 of
+// the first step and the last step's SpMV are dead):
+//   k_bc_first    tiles: r -= alpha q, x += alpha p, z_1 = b_1 l1inv r formed
+//                 at every column it is gathered at, r_1 = r - A z_1, z_2,
+//                 z_cg = z_1 (+ z_2 and the r'z partial when d = 2)
+//   k_bc_step     tiles, k = 2 .. d-1: r_k = r_{k-1} - A z_k, the next z,
+//                 z_cg += z_k (+ z_d and the r'z partial at k = d-1)
 // and a system's rho comes from power-iteration steps (chebyshev.jl:40-54) run
 // by the same launches before its first CG step (phase PH_POWER: w = (A v) .*
 // l1inv and ||w||^2 in k_b_spmv, rho = ||w|| in k_b_fin_delta, v = w / rho in
-// k_bc_init, the step count in k_b_fin_gamma).
+// k_bc_first, the step count in k_b_fin_gamma).
 // The sweep is persistent: a fixed grid, each workgroup walking a contiguous
 // run of the tile table (neighbouring tiles of a system share the workgroup's
 // XCD and its L2); the SpMV takes one tile per workgroup of a large grid.  A
@@ -72,7 +74,7 @@ enum { PH_EMPTY = 0, PH_INIT = 1, PH_ITER = 2, PH_RESID = 3, PH_DONE = 4, PH_POW
 // counted against the LDS counter).
 struct Sys {
     int64_t sptr, col, val, T, D, x, r, z, pa, pb, q, b, part;
-    int64_t l1, l1s, v32, za, zb, rc;  // Chebyshev
+    int64_t l1, l1s, v32, rb, za, zb, rc;  // Chebyshev
     int64_t n, nparts;
     int32_t ntiles, pad;
 };
@@ -88,10 +90,11 @@ struct SysP {
     const double *b;
     double *part;  // two partials per tile
     // Chebyshev: l1inv in FP64 (power iteration) and in the smoother's type,
-    // the FP32 matrix values (FP32 smoother), the z ping-pong and r of the
-    // recurrence (smoother's type)
+    // the FP32 matrix values (FP32 smoother), the CG residual's second buffer
+    // (see cheb_rnew), the z ping-pong and r of the recurrence (smoother's type)
     const double *l1;
     const uint8_t *l1s, *v32;
+    double *rb;
     uint8_t *za, *zb, *rc;
     int64_t n, nparts;
     int32_t ntiles;
@@ -116,6 +119,7 @@ __device__ __forceinline__ SysP load_sys(const Sys *__restrict__ sys, int s, uin
     S.l1 = reinterpret_cast<const double *>(arena + d.l1);
     S.l1s = arena + d.l1s;
     S.v32 = arena + d.v32;
+    S.rb = reinterpret_cast<double *>(arena + d.rb);
     S.za = arena + d.za;
     S.zb = arena + d.zb;
     S.rc = arena + d.rc;
@@ -374,23 +378,39 @@ __device__ __forceinline__ SellView<T> cheb_view(const SysP &S) {
     else return {S.sptr, S.col, reinterpret_cast<const float *>(S.v32), S.n};
 }
 
-// Pointwise start of the preconditioner, one tile per workgroup over the SpMV
-// grid.  Iteration: r -= alpha q, x += alpha p (the CG's own update, as in
-// cg_prologue), z_1 = a_1 0 + b_1 l1inv r (k_c4_init_cg); admission: r = b,
-// x = 0, z_1 from b (k_cg_start, then k_c4_init); power iteration: v = w / rho
-// (k_pow_scale).
+// The CG residual of a Chebyshev system alternates between two buffers,
+// r_k in r[k & 1] (k = the CG iteration; the admission pass writes r_0 = b to
+// r[0]), because k_bc_first gathers r_k at neighbouring rows while it writes
+// r_{k+1}.  The buffer the current pass writes:
+__device__ __forceinline__ double *cheb_rnew(const SysP &S, int ph, long long iter) {
+    return (ph == PH_INIT || (iter & 1)) ? S.r : S.rb;
+}
+
+// Step 1 with the start of the preconditioner folded in, one tile per
+// workgroup over the SpMV grid.  The start is pointwise -- r' = r - alpha q,
+// x += alpha p (the CG's own update, cg_prologue; on admission r' = b,
+// x = 0), z_1 = a_1 0 + b_1 l1inv r' (k_c4_init_cg) -- so z_1 is not stored:
+// r'_j and z_1,j are formed from r, q and l1inv wherever column j is gathered,
+// with the same operations, hence the same values, as the stored z_1 of the
+// single-system kernels.  Then k_c4_step (or, for d = 2, k_c4_step_dot):
+// r_1 = r' - A z_1, z_cg = 0 + z_1, z_2 = a_2 z_1 + b_2 l1inv r_1 (stored for
+// the next step, or added to z_cg with the tile's r'z partial when d = 2).
+// Power iteration: v = w / rho (k_pow_scale).
 template <class T>
-__global__ __launch_bounds__(256) void k_bc_init(const Tile *__restrict__ tiles,
-                                                 const Counts *__restrict__ cnt,
-                                                 const Sys *__restrict__ sys,
-                                                 const State *__restrict__ st,
-                                                 uint8_t *__restrict__ arena) {
+__global__ __launch_bounds__(256) void k_bc_first(const Tile *__restrict__ tiles,
+                                                  const Counts *__restrict__ cnt,
+                                                  const Sys *__restrict__ sys,
+                                                  const State *__restrict__ st,
+                                                  uint8_t *__restrict__ arena, int d) {
+    __shared__ double sm[4];
+    const bool last = (d == 2);
     const int nt = cnt->ntiles;
     int cur = -1, ph = PH_EMPTY;
     SysP S{};
-    T a1 = (T)0, b1 = (T)0;
+    T a1 = (T)0, b1 = (T)0, a2 = (T)0, b2 = (T)0;
     double alpha = 0.0, rho = 1.0;
-    const double *p = nullptr;
+    const double *p = nullptr, *rsrc = nullptr;
+    double *rdst = nullptr;
     bool stop = false;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile tl = tiles[t];
@@ -401,36 +421,59 @@ __global__ __launch_bounds__(256) void k_bc_init(const Tile *__restrict__ tiles,
             rho = st[cur].rho;
             stop = st[cur].pstop != 0;
             cheb_ab<T>(1, rho, a1, b1);
+            cheb_ab<T>(2, rho, a2, b2);
+            const long long it = st[cur].cg.iter;
             alpha = st[cur].cg.gamma / st[cur].cg.delta;
-            p = (st[cur].cg.iter & 1) ? S.pb : S.pa;
+            p = (it & 1) ? S.pb : S.pa;
+            rsrc = (ph == PH_INIT) ? S.b : ((it & 1) ? S.rb : S.r);
+            rdst = cheb_rnew(S, ph, it);
         }
         const int64_t i = (int64_t)tl.tile * TILE + threadIdx.x;
-        if (i >= S.n) continue;
         if (ph == PH_POWER) {
-            if (!stop) S.pa[i] = S.pb[i] / rho;
+            if (i < S.n && !stop) S.pa[i] = S.pb[i] / rho;
             continue;
         }
-        if (ph != PH_INIT && ph != PH_ITER) continue;
-        double ri;
-        if (ph == PH_INIT) {
-            ri = S.b[i];
-            S.x[i] = 0.0;
-        } else {
-            ri = S.r[i] - alpha * S.q[i];
-            S.x[i] = S.x[i] + alpha * p[i];
-        }
-        S.r[i] = ri;
-        const T rt = (T)ri;
+        if (ph != PH_INIT && ph != PH_ITER) continue;  // uniform in the workgroup
+        const bool init = (ph == PH_INIT);
         const T *l1 = reinterpret_cast<const T *>(S.l1s);
-        reinterpret_cast<T *>(S.za)[i] = a1 * (T)0 + b1 * l1[i] * rt;
+        auto rstart = [&](int64_t j) -> double {
+            return init ? rsrc[j] : rsrc[j] - alpha * S.q[j];
+        };
+        auto z1_at = [&](int64_t j) -> T { return a1 * (T)0 + b1 * l1[j] * (T)rstart(j); };
+        double v = 0.0;
+        if (i < S.n) {
+            const double rn = rstart(i);
+            rdst[i] = rn;
+            S.x[i] = init ? 0.0 : S.x[i] + alpha * p[i];
+            const SellView<T> A = cheb_view<T>(S);
+            const int64_t sl = i >> 6;
+            const int64_t e = A.sptr[sl + 1];
+            T ri = (T)rn;
+            for (int64_t q = A.sptr[sl] + (i & 63); q < e; q += 64)
+                ri = ri - A.val[q] * z1_at(A.col[q]);
+            const T zi = z1_at(i);
+            double xi = 0.0 + (double)zi;
+            const T zn = a2 * zi + b2 * l1[i] * ri;
+            if (!last) {
+                reinterpret_cast<T *>(S.rc)[i] = ri;
+                reinterpret_cast<T *>(S.zb)[i] = zn;
+            } else {
+                xi = xi + (double)zn;
+                v = rn * xi;
+            }
+            S.z[i] = xi;
+        }
+        if (last) {
+            v = block_sum(v, sm);
+            if (threadIdx.x == 0) S.part[2 * tl.tile] = v;
+        }
     }
 }
 
-// Step k of d-1 (k_c4_step; the last one is k_c4_step_dot): r_k = r_{k-1} -
-// A z_k with r_0 the CG residual, z_cg = z_1 + ... (z_cg = 0 + z_1 at k = 1),
-// z_{k+1} = a_{k+1} z_k + b_{k+1} l1inv r_k; the last step adds z_d to z_cg
-// instead of storing it and forms the tile's r'z partial.  z_k is read from
-// za (k odd) or zb and z_{k+1} written to the other.
+// Step k of 2 .. d-1 (k_c4_step; the last one is k_c4_step_dot): r_k =
+// r_{k-1} - A z_k, z_cg += z_k, z_{k+1} = a_{k+1} z_k + b_{k+1} l1inv r_k; the
+// last step adds z_d to z_cg instead of storing it and forms the tile's r'z
+// partial.  z_k is read from za (k odd) or zb and z_{k+1} written to the other.
 template <class T>
 __global__ __launch_bounds__(256) void k_bc_step(const Tile *__restrict__ tiles,
                                                  const Counts *__restrict__ cnt,
@@ -443,6 +486,7 @@ __global__ __launch_bounds__(256) void k_bc_step(const Tile *__restrict__ tiles,
     int cur = -1, ph = PH_EMPTY;
     SysP S{};
     T an = (T)0, bn = (T)0;
+    const double *rnew = nullptr;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile tl = tiles[t];
         if (tl.slot != cur) {
@@ -450,6 +494,7 @@ __global__ __launch_bounds__(256) void k_bc_step(const Tile *__restrict__ tiles,
             ph = st[cur].phase;
             S = load_sys(sys, cur, arena);
             cheb_ab<T>(k + 1, st[cur].rho, an, bn);
+            rnew = cheb_rnew(S, ph, st[cur].cg.iter);
         }
         if (ph != PH_INIT && ph != PH_ITER) continue;  // uniform in the workgroup
         const SellView<T> A = cheb_view<T>(S);
@@ -460,17 +505,19 @@ __global__ __launch_bounds__(256) void k_bc_step(const Tile *__restrict__ tiles,
         const int64_t i = (int64_t)tl.tile * TILE + threadIdx.x;
         double v = 0.0;
         if (i < S.n) {
-            const T rin = (k == 1) ? (T)S.r[i] : rc[i];
-            const T ri = row_sub(A, zk, i, rin);
+            const T ri = row_sub(A, zk, i, rc[i]);
             if (!last) rc[i] = ri;
             const T zi = zk[i];
-            double xi = (k == 1) ? 0.0 : S.z[i];
+            double xi = S.z[i];
             xi = xi + (double)zi;
             const T zn = an * zi + bn * l1[i] * ri;
-            if (!last) znext[i] = zn;
-            else xi = xi + (double)zn;
+            if (!last) {
+                znext[i] = zn;
+            } else {
+                xi = xi + (double)zn;
+                v = rnew[i] * xi;
+            }
             S.z[i] = xi;
-            if (last) v = S.r[i] * xi;
         }
         if (last) {
             v = block_sum(v, sm);
@@ -551,7 +598,7 @@ struct Layout {
     int64_t n = 0, nslices = 0, nent = 0, nparts = 0, ntiles = 0, tri = 0;
     int64_t o_sptr = 0, o_col = 0, o_val = 0, o_rlen = 0, o_b = 0, upload = 0;
     int64_t o_T = 0, o_D = 0, o_x = 0, o_r = 0, o_z = 0, o_pa = 0, o_pb = 0, o_q = 0, o_part = 0;
-    int64_t o_l1 = 0, o_l1s = 0, o_v32 = 0, o_za = 0, o_zb = 0, o_rc = 0;
+    int64_t o_l1 = 0, o_l1s = 0, o_v32 = 0, o_rb = 0, o_za = 0, o_zb = 0, o_rc = 0;
     int64_t total = 0;
 };
 
@@ -602,9 +649,11 @@ static Layout make_layout(int64_t n, const int64_t *ptr, const Shape &sh) {
         L.o_l1 = L.total;
         L.o_l1s = f32 ? align256(L.o_l1 + n * 8) : L.o_l1;  // FP64: the same array
         L.o_v32 = align256(L.o_l1s + (f32 ? n * 4 : n * 8));
-        L.o_za = align256(L.o_v32 + (f32 ? L.nent * 4 : 0));
-        L.o_zb = align256(L.o_za + n * tsz);
-        L.o_rc = align256(L.o_zb + n * tsz);
+        L.o_rb = align256(L.o_v32 + (f32 ? L.nent * 4 : 0));
+        // z_1 is never stored; z_2 (zb), z_3 (za), ... from degree 3 on
+        L.o_za = align256(L.o_rb + n * 8);
+        L.o_zb = align256(L.o_za + (sh.degree >= 4 ? n * tsz : 0));
+        L.o_rc = align256(L.o_zb + (sh.degree >= 3 ? n * tsz : 0));
         L.total = align256(L.o_rc + (sh.degree >= 3 ? n * tsz : 0));
     }
     return L;
@@ -941,13 +990,15 @@ void Engine::enqueue_pass(hipStream_t s, unsigned spmv_grid) {
     if (cheb) {
         const dim3 g(spmv_grid), t(TILE);
         if (cfg.precision == HS_PREC_FP64) {
-            hipLaunchKernelGGL(k_bc_init<double>, g, t, 0, s, tl, cn, sy, stt, arena_dev);
-            for (int k = 1; k < cfg.degree; ++k)
+            hipLaunchKernelGGL(k_bc_first<double>, g, t, 0, s, tl, cn, sy, stt, arena_dev,
+                               cfg.degree);
+            for (int k = 2; k < cfg.degree; ++k)
                 hipLaunchKernelGGL(k_bc_step<double>, g, t, 0, s, tl, cn, sy, stt, arena_dev, k,
                                    cfg.degree);
         } else {
-            hipLaunchKernelGGL(k_bc_init<float>, g, t, 0, s, tl, cn, sy, stt, arena_dev);
-            for (int k = 1; k < cfg.degree; ++k)
+            hipLaunchKernelGGL(k_bc_first<float>, g, t, 0, s, tl, cn, sy, stt, arena_dev,
+                               cfg.degree);
+            for (int k = 2; k < cfg.degree; ++k)
                 hipLaunchKernelGGL(k_bc_step<float>, g, t, 0, s, tl, cn, sy, stt, arena_dev, k,
                                    cfg.degree);
         }
@@ -1116,6 +1167,7 @@ void Engine::step() {
             d.l1 = o + L.o_l1;
             d.l1s = o + L.o_l1s;
             d.v32 = o + L.o_v32;
+            d.rb = o + L.o_rb;
             d.za = o + L.o_za;
             d.zb = o + L.o_zb;
             d.rc = o + L.o_rc;
@@ -1315,7 +1367,12 @@ int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out) {
         HS_REQUIRE(S, HS_EINVAL, "no resident system with this id");
         const batch::Layout &L = S->lay;
         const uint8_t *blk = e->arena_dev + S->off;
-        const int64_t off[5] = {L.o_x, L.o_r, L.o_z, L.o_b, L.o_q};
+        int64_t off[5] = {L.o_x, L.o_r, L.o_z, L.o_b, L.o_q};
+        const int64_t slot = S - e->slots.data();
+        batch::State h;
+        if (e->cheb && (what == 1 || what == 8))
+            HS_CHECK_HIP(hipMemcpy(&h, e->d_st.get() + slot, sizeof h, hipMemcpyDeviceToHost));
+        if (e->cheb && what == 1 && (h.cg.iter & 1)) off[1] = L.o_rb;  // r_k is in r[k & 1]
         if (what <= 4) {
             HS_CHECK_HIP(hipMemcpy(out, blk + off[what], L.n * 8, hipMemcpyDeviceToHost));
         } else if (what == 5) {  // D_Dl1 widened to FP64
@@ -1331,9 +1388,6 @@ int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out) {
         } else if (what == 7) {  // l1inv (FP64)
             HS_CHECK_HIP(hipMemcpy(out, blk + L.o_l1, L.n * 8, hipMemcpyDeviceToHost));
         } else {  // rho
-            batch::State h;
-            const int64_t slot = S - e->slots.data();
-            HS_CHECK_HIP(hipMemcpy(&h, e->d_st.get() + slot, sizeof h, hipMemcpyDeviceToHost));
             out[0] = h.rho;
         }
     });

@@ -83,10 +83,11 @@ def iteration_bytes(ptr, partsize: int, precision: str, triangles: int = 2,
     (32 B/row) in the SpMV; with L1GS, the packed triangle(s) as stored
     (`triangles` = 1: L only, U = L^T), D_Dl1, r and x read and written, q
     and p read, z written (56 B/row) in the sweep; with the Chebyshev smoother
-    of degree d (element size e), the start (r, q, x, p, l1inv read; r, x,
-    z_1 written: 48 + 2e B/row) and d-1 steps, each reading the matrix in the
-    smoother's type, z_k, r_{k-1}, l1inv and z_cg and writing z_cg, r_k and
-    z_{k+1} (the last: reading r instead of writing r_k and z_{k+1})."""
+    of degree d (element size e), d-1 steps reading the matrix in the
+    smoother's type: the first reads r, q, l1inv, x and p and writes r, x and
+    z_cg (56 + e B/row), each later one reads z_k, r_{k-1}, l1inv and z_cg and
+    writes z_cg (16 + 3e B/row), and every step but the last writes r_k and
+    z_{k+1} (2e B/row; the last reads r instead, 8 B/row, after the first)."""
     ptr = np.asarray(ptr, dtype=np.int64)
     n = ptr.size - 1
     lens = np.diff(ptr)
@@ -96,11 +97,14 @@ def iteration_bytes(ptr, partsize: int, precision: str, triangles: int = 2,
     spmv = nent * 12 + (w.size + 1) * 8 + 32 * n
     if smoother == "cheb":
         e = 8 if precision == "fp64" else 4
-        tot = spmv + n * (48 + 2 * e)
+        tot = spmv
         for k in range(1, degree):
             last = k == degree - 1
-            row = e + (8 if k == 1 else e) + e + (0 if k == 1 else 8) + 8
-            row += 8 if last else 2 * e
+            row = 56 + e if k == 1 else 16 + 3 * e
+            if not last:
+                row += 2 * e
+            elif k > 1:
+                row += 8
             tot += nent * (e + 4) + (w.size + 1) * 8 + n * row
         return tot
     tsz = {"fp64": 8, "fp32": 4, "bf16": 2}[precision]
