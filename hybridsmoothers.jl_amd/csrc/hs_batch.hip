@@ -979,7 +979,10 @@ static void launch_sweep(int prec, bool symt, unsigned grid, hipStream_t s, cons
 }
 
 void Engine::enqueue_pass(hipStream_t s, unsigned spmv_grid) {
-    const unsigned fin_grid = (unsigned)std::min<int64_t>(cfg.max_systems, 256);
+    // one finalize workgroup per active system up to 1,024 of them (each
+    // system's sum is a dependent chain of loads and barriers; a workgroup
+    // walking several systems pays it once per system)
+    const unsigned fin_grid = (unsigned)std::min<int64_t>(cfg.max_systems, 1024);
     const Tile *tl = d_tiles.get();
     const Counts *cn = d_cnt.get();
     const Sys *sy = d_sys.get();
