@@ -345,7 +345,8 @@ def main():
                     help="only the batched engine's steady state (for kernel profiles)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 192^3 mixed-precision PCG")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB per step (counter calibration)")
+                    help="only time a plain device copy and a sum over MIB MiB per step "
+                         "(counter calibration, read-only stream rate)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the smoother path has no multi-GPU code "
@@ -360,8 +361,13 @@ def main():
         dst = torch.empty_like(src)
         ms = time_kernel(lambda: dst.copy_(src), args.steps, args.warmup,
                          torch.cuda.current_stream())
+        # a read-only stream of the same size: a reduction over the buffer
+        ms_r = time_kernel(lambda: src.sum(), args.steps, args.warmup,
+                           torch.cuda.current_stream())
         print(json.dumps({"copy_bytes_each_way": nb, "ms": ms,
-                          "GB/s_read+write": 2 * nb / (ms * 1e-3) / 1e9}), flush=True)
+                          "GB/s_read+write": 2 * nb / (ms * 1e-3) / 1e9,
+                          "sum_ms": ms_r, "GB/s_read_only": nb / (ms_r * 1e-3) / 1e9}),
+              flush=True)
         return
     import hybridsmoothers as hs
     from hybridsmoothers import problems as pb
