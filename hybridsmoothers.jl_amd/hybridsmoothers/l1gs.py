@@ -334,7 +334,7 @@ class L1GSPreconditioner:
         return self._h
 
     def set_variant(self, variant: int):
-        """Selects the kernel variant of the partsize-64 symmetric sweep (0-4)."""
+        """Selects the kernel variant of the partsize-64 symmetric sweep (0-2)."""
         L.check(L.lib.hs_l1gs_set_variant(self._h, int(variant)))
 
     def save(self, path):
