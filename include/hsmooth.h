@@ -260,9 +260,11 @@ int hs_batch_collect(hs_batch_t e, hs_batch_result *out, int64_t max, int64_t *c
 /* out[0..11] = active, uploading, free slots, free arena bytes, arena bytes,
  * largest free arena block, steps, retired, results waiting, the workgroups
  * of the SpMV (in the last step: the smallest of 1024, 8192, 65536 covering
- * the tile table) and sweep tile kernels, and the packed triangles stored per
+ * the tile table) and sweep tile kernels (0 with the Chebyshev smoother, whose
+ * kernels run on the SpMV's grid), and the packed triangles stored per
  * partition (1: the lower one only -- U = L^T -- whenever the sweep's LDS copy
- * of L fits: partsize 16, partsize 32 and 64 in FP32/BF16; 2: both) */
+ * of L fits: partsize 16, partsize 32 and 64 in FP32/BF16; 2: both; 0 with
+ * the Chebyshev smoother) */
 int hs_batch_info(hs_batch_t e, int64_t *out);
 /* Copies a vector of a resident (submitted, not yet collected) system to host
  * `out` after the last step: what = 0 x, 1 r, 2 z, 3 b, 4 q (n doubles each),
