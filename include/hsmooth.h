@@ -139,7 +139,9 @@ int hs_cheb_smooth(hs_cheb_t s, double *x, const double *b, int loc, void *strea
 int hs_cheb_apply(hs_cheb_t s, double *z, const double *r, int loc, void *stream, int flags);
 /* out = {rho, a*, degree, kind, n} */
 int hs_cheb_info(hs_cheb_t s, double *out);
-/* what: 0 l1inv, 1 r, 2 z, 3 tmp (n doubles each) */
+/* what: 0 l1inv, 1 r, 2 z, 3 tmp (n doubles each); r and z as the last
+ * hs_cheb_smooth or hs_cheb_apply left them (hs_pcg_solve uses them as its
+ * preconditioner's workspace) */
 int hs_cheb_get(hs_cheb_t s, int what, double *out);
 hs_csr_t hs_cheb_matrix(hs_cheb_t s);
 int hs_cheb_destroy(hs_cheb_t s);
