@@ -129,11 +129,12 @@ class BatchSolver:
                  max_systems: int = 1024, staging_bytes: int = 256 << 20, passes: int = 8,
                  grid: int = 0, eta: float = 1.5, triangles: int = 0, device: int = 0,
                  smoother: str = "l1gs", degree: int = 2, power_iters: int = 50):
-        L.init(device)
+        self._h = None
         if precision not in _PREC:
             raise L.HSError(L.HS_EINVAL, f"unknown precision {precision!r}")
         if smoother not in _SMOOTHER:
             raise L.HSError(L.HS_EINVAL, f"unknown smoother {smoother!r}")
+        L.init(device)
         cfg = L.BatchConfig(arena_bytes=int(arena_bytes or default_arena_bytes()),
                             max_systems=int(max_systems), staging_bytes=int(staging_bytes),
                             partsize=int(partsize), precision=_PREC[precision],

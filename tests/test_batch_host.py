@@ -114,3 +114,22 @@ def test_batch_create_argument_errors():
     cnt = C.c_int64(0)
     assert L.lib.hs_batch_collect(None, None, 0, C.byref(cnt)) == L.HS_EINVAL
     assert C.sizeof(L.BatchConfig) == 72 and C.sizeof(L.BatchResult) == 80
+
+
+def test_batch_solver_argument_errors():
+    """The Python engine rejects unknown names before touching the device."""
+    from hybridsmoothers import _lib as L
+    from hybridsmoothers.batch import BatchSolver, iteration_bytes
+    with pytest.raises(L.HSError):
+        BatchSolver(precision="fp16")
+    with pytest.raises(L.HSError):
+        BatchSolver(smoother="jacobi")
+    # the bytes model: Chebyshev degree 2 in FP32 on a 5-point grid is the
+    # SpMV's 12 B/entry plus the smoother's 8 B/entry, 16 B per slice, 92 B/row
+    from hybridsmoothers import problems as pb
+    rp, _, _ = pb.lognormal_grid(64, 64, 1, seed=1)
+    n = rp.size - 1
+    lens = np.diff(rp)
+    w = lens.reshape(-1, 64).max(axis=1)
+    nent = int(64 * w.sum())
+    assert iteration_bytes(rp, 32, "fp32", 1, "cheb", 2) == 20 * nent + 16 * (w.size + 1) + 92 * n
