@@ -73,11 +73,12 @@ __device__ __forceinline__ bool cg_scalars_delta(CGDev &st, double delta) {
 // re-read its last entry), so a wave keeps its loads in flight instead of one
 // dependent index -> value chain per entry; the products are still added one
 // by one in row order.
-__device__ __forceinline__ double cg_spmv_row(const SellView<double> &A,
-                                              const double *__restrict__ z,
-                                              const double *__restrict__ pold,
-                                              double *__restrict__ pnew, double *__restrict__ q,
-                                              bool first, double beta, int64_t i) {
+template <bool C16>
+__device__ __forceinline__ double cg_spmv_row_t(const SellView<double> &A,
+                                                const double *__restrict__ z,
+                                                const double *__restrict__ pold,
+                                                double *__restrict__ pnew, double *__restrict__ q,
+                                                bool first, double beta, int64_t i) {
     constexpr int B = 8;
     const int64_t s = i >> 6;
     const int64_t p0 = A.sptr[s] + (i & 63);
@@ -89,7 +90,8 @@ __device__ __forceinline__ double cg_spmv_row(const SellView<double> &A,
 #pragma unroll
         for (int k = 0; k < B; ++k) {
             const int64_t p = p0 + 64 * (int64_t)min(k0 + k, w - 1);
-            c[k] = A.col[p];
+            if constexpr (C16) c[k] = (int32_t)(i + A.col16[p]);
+            else c[k] = A.col[p];
             v[k] = A.val[p];
         }
 #pragma unroll
@@ -109,6 +111,14 @@ __device__ __forceinline__ double cg_spmv_row(const SellView<double> &A,
     pnew[i] = pi;
     q[i] = acc;
     return pi * acc;
+}
+__device__ __forceinline__ double cg_spmv_row(const SellView<double> &A,
+                                              const double *__restrict__ z,
+                                              const double *__restrict__ pold,
+                                              double *__restrict__ pnew, double *__restrict__ q,
+                                              bool first, double beta, int64_t i) {
+    return A.col16 ? cg_spmv_row_t<true>(A, z, pold, pnew, q, first, beta, i)
+                   : cg_spmv_row_t<false>(A, z, pold, pnew, q, first, beta, i);
 }
 
 // CG vector updates fused into a symmetric L1GS sweep: before the sweep each

@@ -97,14 +97,17 @@ __global__ __launch_bounds__(BLK) void k_sell_width(int64_t n, int64_t nslices,
 }
 
 // slice s, lane l = row 64 s + l: entry k at sptr[s] + 64 k + l, padded with
-// (own row, 0.0); rows past n (the last slice) pad with (n - 1, 0.0)
+// (own row, 0.0); rows past n (the last slice) pad with (n - 1, 0.0).  Also
+// the 16-bit offsets column - row; *wide is set when one does not fit.
 __global__ __launch_bounds__(BLK) void k_sell_pack(int64_t n, int64_t nslices,
                                                    const int64_t *__restrict__ sptr,
                                                    const int64_t *__restrict__ rp1,
                                                    const int64_t *__restrict__ ci1,
                                                    const double *__restrict__ cv,
                                                    int32_t *__restrict__ col,
-                                                   double *__restrict__ val) {
+                                                   double *__restrict__ val,
+                                                   int16_t *__restrict__ col16,
+                                                   int *__restrict__ wide) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t s = i >> 6;
     if (s >= nslices) return;
@@ -113,16 +116,17 @@ __global__ __launch_bounds__(BLK) void k_sell_pack(int64_t n, int64_t nslices,
     const int64_t p0 = i < n ? rp1[i] - 1 : 0;
     const int64_t len = i < n ? rp1[i + 1] - rp1[i] : 0;
     const int32_t pad = (int32_t)(i < n ? i : n - 1);
+    bool fits = true;
     for (int64_t k = 0; k < w; ++k) {
         const int64_t q = sptr[s] + 64 * k + l;
-        if (k < len) {
-            col[q] = (int32_t)(ci1[p0 + k] - 1);
-            val[q] = cv[p0 + k];
-        } else {
-            col[q] = pad;
-            val[q] = 0.0;
-        }
+        const int32_t c = k < len ? (int32_t)(ci1[p0 + k] - 1) : pad;
+        col[q] = c;
+        val[q] = k < len ? cv[p0 + k] : 0.0;
+        const int64_t d = (int64_t)c - i;
+        fits = fits && d >= -32767 && d <= 32767;
+        col16[q] = (int16_t)(fits ? d : 0);
     }
+    if (!fits) atomicOr(wide, 1);
 }
 
 }  // namespace
@@ -211,10 +215,17 @@ void sell_build_device(CSR &A, int64_t n, int64_t nnz, const int64_t *rp1, const
     HS_CHECK_HIP(hipStreamSynchronize(st));
     A.col.alloc(A.nent);
     A.val.alloc(A.nent);
+    A.col16.alloc(A.nent);
+    DBuf<int> wide(1);
+    HS_CHECK_HIP(hipMemsetAsync(wide.get(), 0, sizeof(int), st));
     hipLaunchKernelGGL(k_sell_pack, dim3(grid_for(rows, BLK)), dim3(BLK), 0, st, n, A.nslices,
-                       A.sptr.get(), rp1, ci1, cv, A.col.get(), A.val.get());
+                       A.sptr.get(), rp1, ci1, cv, A.col.get(), A.val.get(), A.col16.get(),
+                       wide.get());
     check_launch();
+    int h_wide = 0;
+    HS_CHECK_HIP(hipMemcpyAsync(&h_wide, wide.get(), sizeof(int), hipMemcpyDeviceToHost, st));
     HS_CHECK_HIP(hipStreamSynchronize(st));  // the caller's inputs may go out of scope
+    if (h_wide) A.col16.release();  // some column is farther than 32,767 from its row
 }
 
 void csr_create_device(CSR &A, int64_t n, const int64_t *ptr, const int64_t *ind,

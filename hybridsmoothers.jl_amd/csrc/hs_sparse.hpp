@@ -15,6 +15,10 @@ namespace hs {
 // (column = the row itself, value = 0).  One wave reads one slice with fully
 // coalesced loads, and every row is still summed in its CSR order (padding
 // comes last and adds zero), so results equal a row-by-row CSR product.
+// When every stored column is within 32,767 of its row (banded and stencil
+// matrices up to that bandwidth), the products read the column as a 16-bit
+// offset from the row (col16: 2 bytes per entry instead of 4); the 32-bit
+// columns are kept for the read-back and for matrices where it does not fit.
 // The operator is built on the device (hs_setup.hip); the row lengths are
 // kept so that the CSR can be read back (csr_download).
 // ---------------------------------------------------------------------------
@@ -27,6 +31,7 @@ struct SellView {
     const int32_t *col;
     const T *val;
     int64_t n;
+    const int16_t *col16 = nullptr;  // column - row, when every entry fits
 };
 
 struct CSR {
@@ -37,6 +42,7 @@ struct CSR {
     DBuf<double> val;
     DBuf<float> val32;
     DBuf<int32_t> rowlen;
+    DBuf<int16_t> col16;
     CGWork *cg = nullptr;
 
     CSR() = default;
@@ -46,14 +52,18 @@ struct CSR {
         if (cg) cgwork_free(cg);
     }
     void ensure_fp32(hipStream_t st);
-    SellView<double> view() const { return {sptr.get(), col.get(), val.get(), n}; }
-    SellView<float> view32() const { return {sptr.get(), col.get(), val32.get(), n}; }
+    SellView<double> view() const { return {sptr.get(), col.get(), val.get(), n, col16.get()}; }
+    SellView<float> view32() const {
+        return {sptr.get(), col.get(), val32.get(), n, col16.get()};
+    }
     template <class T> SellView<T> view_as() const {
         if constexpr (sizeof(T) == 8) return view();
         else return view32();
     }
     // algorithmic bytes of one product: values, column indices, slice offsets
-    int64_t bytes(int vbytes) const { return nent * (vbytes + 4) + (nslices + 1) * 8; }
+    int64_t bytes(int vbytes) const {
+        return nent * (vbytes + (col16.get() ? 2 : 4)) + (nslices + 1) * 8;
+    }
 };
 
 // Device setup (hs_setup.hip).  Arrays are 1-based Int64 as SparseMatrixCSC /
@@ -80,13 +90,25 @@ void csr_download(const CSR &A, int64_t *rowptr1, int64_t *colval1, double *nzva
 // ---------------------------------------------------------------------------
 // row products, summed in row order
 // ---------------------------------------------------------------------------
-template <class T, class TX>
-__device__ __forceinline__ T row_dot(const SellView<T> &A, const TX *__restrict__ x, int64_t i) {
+// Calls f(value, column) for the stored entries of row i in order (padding
+// last); the column format is the same for the whole matrix, so the branch
+// is uniform.
+template <class T, class F>
+__device__ __forceinline__ void row_entries(const SellView<T> &A, int64_t i, F &&f) {
     const int64_t s = i >> 6;
     const int l = (int)(i & 63);
     const int64_t e = A.sptr[s + 1];
+    if (A.col16) {
+        for (int64_t p = A.sptr[s] + l; p < e; p += 64) f(A.val[p], i + A.col16[p]);
+    } else {
+        for (int64_t p = A.sptr[s] + l; p < e; p += 64) f(A.val[p], (int64_t)A.col[p]);
+    }
+}
+
+template <class T, class TX>
+__device__ __forceinline__ T row_dot(const SellView<T> &A, const TX *__restrict__ x, int64_t i) {
     T acc = (T)0;
-    for (int64_t p = A.sptr[s] + l; p < e; p += 64) acc = acc + A.val[p] * (T)x[A.col[p]];
+    row_entries(A, i, [&](T a, int64_t c) { acc = acc + a * (T)x[c]; });
     return acc;
 }
 
@@ -94,10 +116,7 @@ __device__ __forceinline__ T row_dot(const SellView<T> &A, const TX *__restrict_
 template <class T, class TX>
 __device__ __forceinline__ T row_sub(const SellView<T> &A, const TX *__restrict__ x, int64_t i,
                                      T r) {
-    const int64_t s = i >> 6;
-    const int l = (int)(i & 63);
-    const int64_t e = A.sptr[s + 1];
-    for (int64_t p = A.sptr[s] + l; p < e; p += 64) r = r - A.val[p] * (T)x[A.col[p]];
+    row_entries(A, i, [&](T a, int64_t c) { r = r - a * (T)x[c]; });
     return r;
 }
 

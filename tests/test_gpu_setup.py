@@ -122,3 +122,31 @@ def test_invalid_sparse_arrays_rejected(gpu, bad):
                              C.byref(h))
     assert st == L.HS_EINVAL, bad
     assert "invalid sparse matrix" in L.lib.hs_last_error().decode()
+
+
+@pytest.mark.parametrize("far", [False, True])
+def test_operator_column_formats(gpu, far):
+    """Products are the oracle's bit for bit whether the operator stores 16-bit
+    column offsets (every column within 32,767 of its row) or, with one
+    coupling 39,999 apart, the 32-bit columns."""
+    import oracle as O
+    from hybridsmoothers.chebyshev import csr_download
+    hs = gpu
+    n = 40000
+    rng = np.random.default_rng(7)
+    A = sp.diags([rng.uniform(-1, 1, n - k) for k in (1, 300, 20000)], [1, 300, 20000],
+                 shape=(n, n), format="lil")
+    if far:
+        A[0, n - 1] = 0.5
+    A = (A + A.T + sp.identity(n) * 4.0).tocsc()
+    A.sort_indices()
+    x = rng.uniform(-1, 1, n)
+    D = hs.DeviceMatrix(A)
+    y = np.zeros(n)
+    D.mul(y, x)
+    np.testing.assert_array_equal(y, O.spmv(O.csr1(A), x))
+    B = csr_download(D.handle)
+    R = A.tocsr()
+    R.sort_indices()
+    np.testing.assert_array_equal(B.indices, R.indices)
+    np.testing.assert_array_equal(B.data, R.data)
