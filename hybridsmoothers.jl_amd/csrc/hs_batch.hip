@@ -76,13 +76,14 @@ struct Sys {
     int64_t sptr, col, val, T, D, x, r, z, pa, pb, q, b, part;
     int64_t l1, l1s, v32, rb, za, zb, rc;  // Chebyshev
     int64_t n, nparts;
-    int32_t ntiles, pad;
+    int32_t ntiles, c16;  // c16: columns stored as 16-bit offsets from the row
 };
 
 // The same, resolved against the arena base.
 struct SysP {
     const int64_t *sptr;
     const int32_t *col;
+    const int16_t *col16;  // the column section as 16-bit offsets, or null
     const double *val;
     const uint8_t *T;  // packed triangles (storage type of the engine's precision)
     const uint8_t *D;  // D_Dl1 (FP64, or FP32 for the reduced-precision smoothers)
@@ -105,6 +106,7 @@ __device__ __forceinline__ SysP load_sys(const Sys *__restrict__ sys, int s, uin
     SysP S;
     S.sptr = reinterpret_cast<const int64_t *>(arena + d.sptr);
     S.col = reinterpret_cast<const int32_t *>(arena + d.col);
+    S.col16 = d.c16 ? reinterpret_cast<const int16_t *>(arena + d.col) : nullptr;
     S.val = reinterpret_cast<const double *>(arena + d.val);
     S.T = arena + d.T;
     S.D = arena + d.D;
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(256) void k_b_spmv(const Tile *__restrict__ tiles,
             pnew = (k & 1) ? S.pb : S.pa;
         }
         if (ph != PH_ITER && ph != PH_RESID && ph != PH_POWER) continue;  // uniform in the workgroup
-        const SellView<double> A{S.sptr, S.col, S.val, S.n};
+        const SellView<double> A{S.sptr, S.col, S.val, S.n, S.col16};
         const int64_t i = (int64_t)tl.tile * TILE + threadIdx.x;
         if (ph == PH_ITER) {
             double d = 0.0;
@@ -374,8 +376,8 @@ __device__ __forceinline__ void cheb_ab(int k, double rho, T &ak, T &bk) {
 
 template <class T>
 __device__ __forceinline__ SellView<T> cheb_view(const SysP &S) {
-    if constexpr (sizeof(T) == 8) return {S.sptr, S.col, S.val, S.n};
-    else return {S.sptr, S.col, reinterpret_cast<const float *>(S.v32), S.n};
+    if constexpr (sizeof(T) == 8) return {S.sptr, S.col, S.val, S.n, S.col16};
+    else return {S.sptr, S.col, reinterpret_cast<const float *>(S.v32), S.n, S.col16};
 }
 
 // The CG residual of a Chebyshev system alternates between two buffers,
@@ -446,11 +448,8 @@ __global__ __launch_bounds__(256) void k_bc_first(const Tile *__restrict__ tiles
             rdst[i] = rn;
             S.x[i] = init ? 0.0 : S.x[i] + alpha * p[i];
             const SellView<T> A = cheb_view<T>(S);
-            const int64_t sl = i >> 6;
-            const int64_t e = A.sptr[sl + 1];
             T ri = (T)rn;
-            for (int64_t q = A.sptr[sl] + (i & 63); q < e; q += 64)
-                ri = ri - A.val[q] * z1_at(A.col[q]);
+            row_entries(A, i, [&](T a, int64_t c) { ri = ri - a * z1_at(c); });
             const T zi = z1_at(i);
             double xi = 0.0 + (double)zi;
             const T zn = a2 * zi + b2 * l1[i] * ri;
@@ -531,7 +530,8 @@ __global__ __launch_bounds__(256) void k_bc_step(const Tile *__restrict__ tiles,
 // k_setup_rows does for a CSR or symmetric CSC).
 template <class TS, class TC>
 __global__ void k_b_setup(int64_t n, int64_t P, const int64_t *__restrict__ sptr,
-                          const int32_t *__restrict__ col, const double *__restrict__ val,
+                          const int32_t *__restrict__ col, const int16_t *__restrict__ col16,
+                          const double *__restrict__ val,
                           const int32_t *__restrict__ rlen, double eta, TC *__restrict__ D,
                           TS *__restrict__ T, int mode) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -548,7 +548,7 @@ __global__ void k_b_setup(int64_t n, int64_t P, const int64_t *__restrict__ sptr
     const int len = rlen[g];
     for (int kk = 0; kk < len; ++kk) {
         const int64_t p = base + 64 * (int64_t)kk;
-        const int64_t j = col[p];
+        const int64_t j = col16 ? g + col16[p] : (int64_t)col[p];
         const double v = val[p];
         if (j == g) {
             a = v;
@@ -661,12 +661,24 @@ static Layout make_layout(int64_t n, const int64_t *ptr, const Shape &sh) {
 
 // Sliced-ELL of the rows (1-based CSR, or the columns of a symmetric CSC),
 // row lengths and b, written into `dst` at the layout's offsets.  The slice
-// format is the device operator's (padding: the row itself, value 0).
-static void fill_upload(const Layout &L, const int64_t *ptr, const int64_t *ind,
+// format is the device operator's (padding: the row itself, value 0); the
+// columns are stored as 16-bit offsets from the row when every one fits
+// (returns true; the column section then holds int16 values), else as int32.
+static bool fill_upload(const Layout &L, const int64_t *ptr, const int64_t *ind,
                         const double *val, const double *b, uint8_t *dst) {
     const int64_t n = L.n;
+    bool c16 = true;
+    for (int64_t i = 0; i < n && c16; ++i)
+        for (int64_t p = ptr[i] - 1; p < ptr[i + 1] - 1; ++p) {
+            const int64_t d = ind[p] - 1 - i;
+            if (d < -32767 || d > 32767) {
+                c16 = false;
+                break;
+            }
+        }
     int64_t *sp = reinterpret_cast<int64_t *>(dst + L.o_sptr);
     int32_t *col = reinterpret_cast<int32_t *>(dst + L.o_col);
+    int16_t *col16 = reinterpret_cast<int16_t *>(dst + L.o_col);
     double *v = reinterpret_cast<double *>(dst + L.o_val);
     int32_t *rl = reinterpret_cast<int32_t *>(dst + L.o_rlen);
     sp[0] = 0;
@@ -683,19 +695,22 @@ static void fill_upload(const Layout &L, const int64_t *ptr, const int64_t *ind,
             if (row) rl[i] = (int32_t)len;
             for (int64_t k = 0; k < w; ++k) {
                 const int64_t q = sp[s] + 64 * k + l;
+                int64_t c;
                 if (k < len) {
-                    const int64_t c = ind[p0 + k] - 1;
+                    c = ind[p0 + k] - 1;
                     HS_REQUIRE(c >= 0 && c < n, HS_EINVAL, "column index out of range");
-                    col[q] = (int32_t)c;
                     v[q] = val[p0 + k];
                 } else {
-                    col[q] = (int32_t)(row ? i : n - 1);
+                    c = row ? i : n - 1;
                     v[q] = 0.0;
                 }
+                if (c16) col16[q] = (int16_t)(c - i);
+                else col[q] = (int32_t)c;
             }
         }
     }
     std::memcpy(dst + L.o_b, b, n * 8);
+    return c16;
 }
 
 // First-fit allocator of arena offsets with coalescing of free neighbours.
@@ -763,6 +778,7 @@ struct Slot {
     int64_t itmax = 0;
     double t_submit = 0, t_start = 0;
     hipEvent_t ready = nullptr;
+    bool c16 = false;  // columns uploaded as 16-bit offsets
 };
 
 struct Engine {
@@ -1042,13 +1058,14 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
     int64_t off = 0;
     if (!arena.alloc(L.total, off)) return HS_EAGAIN;
     const int s = free_slots.back();
+    bool c16 = false;
     try {
         if (stage_off + L.upload > cfg.staging_bytes) {
             HS_CHECK_HIP(hipStreamSynchronize(us));  // earlier uploads have left the staging buffer
             stage_off = 0;
         }
         uint8_t *src = staging + stage_off;
-        fill_upload(L, ptr, ind, val, b, src);
+        c16 = fill_upload(L, ptr, ind, val, b, src);
         stage_off = align256(stage_off + L.upload);
         uint8_t *blk = arena_dev + off;
         HS_CHECK_HIP(hipMemcpyAsync(blk, src, L.upload, hipMemcpyHostToDevice, us));
@@ -1057,6 +1074,7 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
         const int mode = one_tri ? MODE_SYMT : MODE_SYM;
         const auto *sp = reinterpret_cast<const int64_t *>(blk + L.o_sptr);
         const auto *cl = reinterpret_cast<const int32_t *>(blk + L.o_col);
+        const auto *c16p = c16 ? reinterpret_cast<const int16_t *>(blk + L.o_col) : nullptr;
         const auto *vl = reinterpret_cast<const double *>(blk + L.o_val);
         const auto *rl = reinterpret_cast<const int32_t *>(blk + L.o_rlen);
         if (cheb) {
@@ -1074,15 +1092,15 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
                                    1.0 / std::sqrt((double)n));
         } else if (cfg.precision == HS_PREC_FP64)
             hipLaunchKernelGGL((k_b_setup<double, double>), g, t, 0, us, n, cfg.partsize, sp, cl,
-                               vl, rl, cfg.eta, reinterpret_cast<double *>(blk + L.o_D),
+                               c16p, vl, rl, cfg.eta, reinterpret_cast<double *>(blk + L.o_D),
                                reinterpret_cast<double *>(blk + L.o_T), mode);
         else if (cfg.precision == HS_PREC_FP32)
-            hipLaunchKernelGGL((k_b_setup<float, float>), g, t, 0, us, n, cfg.partsize, sp, cl, vl,
-                               rl, cfg.eta, reinterpret_cast<float *>(blk + L.o_D),
+            hipLaunchKernelGGL((k_b_setup<float, float>), g, t, 0, us, n, cfg.partsize, sp, cl,
+                               c16p, vl, rl, cfg.eta, reinterpret_cast<float *>(blk + L.o_D),
                                reinterpret_cast<float *>(blk + L.o_T), mode);
         else
             hipLaunchKernelGGL((k_b_setup<__hip_bfloat16, float>), g, t, 0, us, n, cfg.partsize,
-                               sp, cl, vl, rl, cfg.eta, reinterpret_cast<float *>(blk + L.o_D),
+                               sp, cl, c16p, vl, rl, cfg.eta, reinterpret_cast<float *>(blk + L.o_D),
                                reinterpret_cast<__hip_bfloat16 *>(blk + L.o_T), mode);
         check_launch();
         HS_CHECK_HIP(hipEventRecord(slots[s].ready, us));
@@ -1096,6 +1114,7 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
     S.id = next_id++;
     S.off = off;
     S.lay = L;
+    S.c16 = c16;
     S.x_out = x_out;
     S.rtol = rtol;
     S.atol = atol;
@@ -1177,6 +1196,7 @@ void Engine::step() {
             d.n = L.n;
             d.nparts = L.nparts;
             d.ntiles = (int32_t)L.ntiles;
+            d.c16 = S.c16 ? 1 : 0;
             State &h = h_st[s];
             std::memset(&h, 0, sizeof(State));
             h.cg.atol = S.atol;

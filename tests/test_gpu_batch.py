@@ -198,3 +198,38 @@ def test_autotune(gpu):
     x = torch.tensor(pb.uniform_rhs(A.shape[0], 1), device="cuda")
     times = autotune.tune_l1gs_variant(P, x, reps=5)
     assert set(times) == {0, 1, 2}
+
+
+@pytest.mark.parametrize("smoother", ["l1gs", "cheb"])
+def test_batch_column_formats(gpu, smoother):
+    """A system whose columns all lie within 32,767 of their rows (uploaded as
+    16-bit offsets) and one with a coupling 39,999 apart (32-bit columns),
+    solved together: each equals the single-system solve bit for bit."""
+    import scipy.sparse as sp
+    from hybridsmoothers.batch import BatchSolver
+    hs = gpu
+    n = 40000
+    rng = np.random.default_rng(8)
+    systems = []
+    for far in (False, True):
+        A = sp.diags([rng.uniform(-1, 1, n - k) for k in (1, 300, 20000)], [1, 300, 20000],
+                     shape=(n, n), format="lil")
+        if far:
+            A[0, n - 1] = 0.5
+        A = (A + A.T + sp.identity(n) * 8.0).tocsc()
+        A.sort_indices()
+        systems.append((A, rng.uniform(-1, 1, n)))
+    kw = (dict(smoother="cheb", degree=2) if smoother == "cheb" else dict(partsize=32))
+    with BatchSolver(precision="fp64", passes=8, arena_bytes=256 << 20, max_systems=4,
+                     **kw) as eng:
+        sols = eng.solve_all(systems, rtol=1e-8)
+    for (A, b), s in zip(systems, sols):
+        if smoother == "cheb":
+            Pl = hs._build_smoother(hs.ChebyshevFourth(degree=2, backend=hs.ROCBackend()), A)
+        else:
+            Pl = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=20)(
+                A, 32, isSymA=True, sweep=hs.SymmetricSweep(), precision="fp64")
+        ref = hs.pcg_solve(A, b, Pl=Pl, reltol=1e-8)
+        assert s.status == "solved" and s.iters == ref.iters
+        np.testing.assert_array_equal(s.x, ref.u)
+        assert s.true_relres == ref.true_relres
