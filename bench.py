@@ -278,7 +278,7 @@ def batch_steady(kind, nsys=256, iters=60):
     systems (2D 181^2 or 3D 32^3 lognormal, 8.4 M rows in all) resident
     together, FP32 L1GS partsize 32, a fixed `iters` CG iterations."""
     from hybridsmoothers import problems as pb
-    from hybridsmoothers.batch import BatchSolver, iteration_bytes
+    from hybridsmoothers.batch import BatchSolver, column_bytes, iteration_bytes
     dims = (181, 181, 1) if kind == "2d" else (32, 32, 32)
     systems = []
     for i in range(nsys):
@@ -294,7 +294,8 @@ def batch_steady(kind, nsys=256, iters=60):
             eng.step()
             sols += eng.collect()
         dt = time.perf_counter() - t0
-        ib = iteration_bytes(systems[0][0][0], 32, "fp32", eng.triangles)
+        A0 = systems[0][0]
+        ib = iteration_bytes(A0[0], 32, "fp32", eng.triangles, cbytes=column_bytes(A0[0], A0[1]))
     rows = nsys * systems[0][1].size
     return {"workload": f"{nsys} x lognormal {'x'.join(map(str, dims))}", "rows": rows,
             "iterations": sols[0].iters, "bytes_per_iteration": ib * nsys, "wall_s": round(dt, 4),

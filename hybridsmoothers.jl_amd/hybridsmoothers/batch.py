@@ -75,11 +75,22 @@ def csr_arrays(A):
             np.ascontiguousarray(A.data, dtype=np.float64))
 
 
+def column_bytes(ptr, ind) -> int:
+    """Bytes per stored column in the engine: 2 when every column of the
+    1-based arrays lies within 32,767 of its row (uploaded as 16-bit
+    offsets), else 4."""
+    ptr = np.asarray(ptr, dtype=np.int64)
+    ind = np.asarray(ind, dtype=np.int64)
+    rows = np.repeat(np.arange(ptr.size - 1, dtype=np.int64), np.diff(ptr))
+    d = ind - 1 - rows
+    return 2 if d.size == 0 or int(np.abs(d).max()) <= 32767 else 4
+
+
 def iteration_bytes(ptr, partsize: int, precision: str, triangles: int = 2,
-                    smoother: str = "l1gs", degree: int = 2) -> int:
+                    smoother: str = "l1gs", degree: int = 2, cbytes: int = 4) -> int:
     """Algorithmic HBM bytes of one fused CG iteration of one system in the
-    engine: the sliced-ELL matrix (8-byte value + 4-byte column per stored
-    entry, slice offsets), z and p_{k-1} gathered once and p_k, q written
+    engine: the sliced-ELL matrix (8-byte value + a `cbytes`-byte column per
+    stored entry (column_bytes), slice offsets), z and p_{k-1} gathered once and p_k, q written
     (32 B/row) in the SpMV; with L1GS, the packed triangle(s) as stored
     (`triangles` = 1: L only, U = L^T), D_Dl1, r and x read and written, q
     and p read, z written (56 B/row) in the sweep; with the Chebyshev smoother
@@ -94,7 +105,7 @@ def iteration_bytes(ptr, partsize: int, precision: str, triangles: int = 2,
     pad = (-n) % 64
     w = np.concatenate([lens, np.zeros(pad, np.int64)]).reshape(-1, 64).max(axis=1)
     nent = int(64 * w.sum())
-    spmv = nent * 12 + (w.size + 1) * 8 + 32 * n
+    spmv = nent * (8 + cbytes) + (w.size + 1) * 8 + 32 * n
     if smoother == "cheb":
         e = 8 if precision == "fp64" else 4
         tot = spmv
@@ -105,7 +116,7 @@ def iteration_bytes(ptr, partsize: int, precision: str, triangles: int = 2,
                 row += 2 * e
             elif k > 1:
                 row += 8
-            tot += nent * (e + 4) + (w.size + 1) * 8 + n * row
+            tot += nent * (e + cbytes) + (w.size + 1) * 8 + n * row
         return tot
     tsz = {"fp64": 8, "fp32": 4, "bf16": 2}[precision]
     dsz = 8 if precision == "fp64" else 4

@@ -32,7 +32,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import problems as pb
-from .batch import BatchSolver, iteration_bytes
+from .batch import BatchSolver, column_bytes, iteration_bytes
 
 
 def poisson_arrivals(count: int, rate: float, seed: int = 0) -> np.ndarray:
@@ -197,9 +197,10 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
                 del staged[i]
                 pending.pop(k)
                 rows += b.size
-                ibytes[i] = iteration_bytes(A[0] if isinstance(A, tuple) else A.indptr + 1,
-                                            engine.partsize, engine.precision, engine.triangles,
-                                            engine.smoother, engine.degree)
+                ptr, ind = (A[0], A[1]) if isinstance(A, tuple) else (A.indptr + 1, A.indices + 1)
+                ibytes[i] = iteration_bytes(ptr, engine.partsize, engine.precision,
+                                            engine.triangles, engine.smoother, engine.degree,
+                                            column_bytes(ptr, ind))
             if busy:
                 for s in engine.collect():
                     i = s.tag

@@ -125,11 +125,18 @@ def test_batch_solver_argument_errors():
     with pytest.raises(L.HSError):
         BatchSolver(smoother="jacobi")
     # the bytes model: Chebyshev degree 2 in FP32 on a 5-point grid is the
-    # SpMV's 12 B/entry plus the smoother's 8 B/entry, 16 B per slice, 92 B/row
+    # SpMV's 8 + c B/entry plus the smoother's 4 + c B/entry (c the column
+    # bytes: 2 on this grid), 16 B per slice, 92 B/row
     from hybridsmoothers import problems as pb
-    rp, _, _ = pb.lognormal_grid(64, 64, 1, seed=1)
+    from hybridsmoothers.batch import column_bytes
+    rp, ci, _ = pb.lognormal_grid(64, 64, 1, seed=1)
     n = rp.size - 1
     lens = np.diff(rp)
     w = lens.reshape(-1, 64).max(axis=1)
     nent = int(64 * w.sum())
-    assert iteration_bytes(rp, 32, "fp32", 1, "cheb", 2) == 20 * nent + 16 * (w.size + 1) + 92 * n
+    c = column_bytes(rp, ci)
+    assert c == 2
+    assert iteration_bytes(rp, 32, "fp32", 1, "cheb", 2, c) == \
+        (12 + 2 * c) * nent + 16 * (w.size + 1) + 92 * n
+    # a coupling 40,000 rows away needs 32-bit columns
+    assert column_bytes(np.array([1, 2, 3]), np.array([40001, 1])) == 4
