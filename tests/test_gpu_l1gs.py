@@ -314,3 +314,48 @@ def test_c3_convdiff2048_parity(gpu):
             np.testing.assert_array_equal(y, yo)
         else:
             assert relerr(y, yo) < TOL
+
+
+@pytest.mark.parametrize("partsize", [16, 32, 64, 128])
+def test_unsafe_diagonal_true_division(gpu, partsize):
+    """Rows whose l1 diagonal lies outside 2^+-400 (here ~1e130, the matrix
+    scaled symmetrically by 1e65 on a few rows and columns) make their wave
+    divide instead of multiplying by a reciprocal.  Every kernel that path
+    reaches -- packed forward / backward / symmetric (all P = 64 variants), the
+    one-triangle layout, the matrix view -- against the oracle element by
+    element (tolerance relative to each entry, so the ~1e-130 entries count)."""
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.csr_to_scipy(*pb.lognormal_grid(20, 20, 20, seed=3)).tocsc()
+    n = A.shape[0]
+    s = np.ones(n)
+    s[[5, 700, 4001, n - 1]] = 1e65
+    S = sp.diags(s)
+    A = (S @ A @ S).tocsc()
+    A.sort_indices()
+    x = np.random.default_rng(partsize).uniform(-1, 1, n)
+    print(f"[unsafe diagonal] n={n} partsize={partsize}", flush=True)
+    b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=64)
+    cases = [("symmetric", "packed", True), ("forward", "packed", False),
+             ("backward", "packed", False), ("symmetric", "view", True)]
+    if partsize <= 64:
+        cases.append(("symmetric", "packed_sym", True))
+    for sw, cache, sym in cases:
+        ref = O.l1gs_build(A, partsize, is_sym=sym, sweep=sw,
+                           cache="view" if cache == "view" else "packed", gpu=(256, 64))
+        D = ref.D
+        assert np.abs(D).max() > 2.0 ** 400
+        yo = O.l1gs_ldiv(ref, x)
+        cs = {"packed": hs.PackedBufferCache(), "view": hs.MatrixViewCache(),
+              "packed_sym": hs.PackedSymmetricCache()}[cache]
+        P = b(A, partsize, isSymA=sym, sweep=_hs_sweep(hs, sw), cache_strategy=cs)
+        variants = (0, 1, 2) if (partsize == 64 and sw == "symmetric" and cache == "packed") else (None,)
+        for v in variants:
+            if v is not None:
+                P.set_variant(v)
+            y = hs.backslash(P, x)
+            if sw == "forward":
+                np.testing.assert_array_equal(y, yo)
+            else:
+                np.testing.assert_allclose(y, yo, rtol=1e-11, atol=0, err_msg=f"{sw} {cache} {v}")
