@@ -302,9 +302,10 @@ def batch_steady(kind, nsys=256, iters=60):
             "GB/s_whole_run": round(ib * nsys * (sols[0].iters + 1) / dt / 1e9, 1)}
 
 
-def serving_cpu_baseline(budget_s=10.0):
-    """The oracle's PCG with its L1GS (FP64, partsize 32, OpenMP) on systems of
-    the serving stream, in stream order, until the time budget is spent."""
+def serving_cpu_baseline(budget_s=10.0, smoother="l1gs"):
+    """The oracle's PCG with its L1GS (FP64, partsize 32) or its Chebyshev 4th
+    kind of degree 2 (FP64, 50 power-iteration steps), with OpenMP, on systems
+    of the serving stream, in stream order, until the time budget is spent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from hybridsmoothers import problems as pb
@@ -314,15 +315,19 @@ def serving_cpu_baseline(budget_s=10.0):
     for i, spec in enumerate(specs):
         A = pb.csr_to_scipy(*pb.stream_system(spec))
         b = pb.uniform_rhs(A.shape[0], 1000 + i)
-        ref = O.l1gs_build(A, 32, is_sym=True, sweep="symmetric", nthreads=threads)
+        if smoother == "cheb":
+            ref = O.cheb_build(A, 4, 2, nthreads=threads)
+        else:
+            ref = O.l1gs_build(A, 32, is_sym=True, sweep="symmetric", nthreads=threads)
         O.pcg(A, b, ref, rtol=1e-8, nthreads=threads)
         done += 1
         rows += A.shape[0]
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
+    what = "Chebyshev 4th kind d=2" if smoother == "cheb" else "L1GS P=32"
     return {"value": round(done / dt, 3), "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} systems of the stream ({rows} rows), oracle L1GS P=32 FP64 "
+            "sample": f"first {done} systems of the stream ({rows} rows), oracle {what} FP64 "
                       f"+ CG with OpenMP, build included, {dt:.1f} s"}
 
 
@@ -381,6 +386,8 @@ def main():
         res = serving_bench(args)
         if not args.no_cpu:
             res["cpu_baseline"] = serving_cpu_baseline()
+            if "chebyshev_d2" in res:
+                res["chebyshev_d2"]["cpu_baseline"] = serving_cpu_baseline(smoother="cheb")
         print(json.dumps({"serving": res}), flush=True)
         return
     g, P = args.grid, args.partsize
@@ -548,6 +555,9 @@ def main():
         if not args.no_cpu:
             log("[bench] serving CPU baseline (oracle)")
             out["serving"]["cpu_baseline"] = serving_cpu_baseline()
+            if "chebyshev_d2" in out["serving"]:
+                out["serving"]["chebyshev_d2"]["cpu_baseline"] = serving_cpu_baseline(
+                    smoother="cheb")
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(A, P)
