@@ -1079,7 +1079,10 @@ int Engine::submit(int64_t n, const int64_t *ptr, const int64_t *ind, const doub
         const auto *rl = reinterpret_cast<const int32_t *>(blk + L.o_rlen);
         if (cheb) {
             const bool f32 = cfg.precision == HS_PREC_FP32;
-            hipLaunchKernelGGL(k_bc_setup, g, t, 0, us, SellView<double>{sp, cl, vl, n},
+            // the view of the uploaded format: 16-bit offsets from the row, or
+            // 32-bit columns (never the 16-bit section read as 32-bit columns)
+            const SellView<double> av{sp, c16p ? nullptr : cl, vl, n, c16p};
+            hipLaunchKernelGGL(k_bc_setup, g, t, 0, us, av,
                                reinterpret_cast<double *>(blk + L.o_l1),
                                f32 ? reinterpret_cast<float *>(blk + L.o_l1s) : nullptr);
             if (f32)

@@ -263,8 +263,8 @@ __device__ __forceinline__ TC sweep_ring64(TC xr, TC dr, TC rr, int lane,
     auto issue_at = [&](auto T) {
         constexpr int t = decltype(T)::value;
         constexpr int c0 = kth_issue(SC, t, 0), c1 = kth_issue(SC, t, 1);
-        if constexpr (c0 >= 0) chunk[c0] = blk[64 * c0 + lane];
-        if constexpr (c1 >= 0) chunk[c1] = blk[64 * c1 + lane];
+        if constexpr (c0 >= 0) chunk[c0] = ld_stream(blk + 64 * c0 + lane);
+        if constexpr (c1 >= 0) chunk[c1] = ld_stream(blk + 64 * c1 + lane);
     };
     // LDS writes of the chunks first read at step t
     auto write_at = [&](auto T) {
@@ -291,7 +291,7 @@ __device__ __forceinline__ TC sweep_ring64(TC xr, TC dr, TC rr, int lane,
     // prologue: the loads first used within RUN steps, group 0 into LDS and registers
     static_for<0, count_issue0(SC)>([&](auto K) {
         constexpr int c = kth_issue(SC, 0, decltype(K)::value);
-        chunk[c] = blk[64 * c + lane];
+        chunk[c] = ld_stream(blk + 64 * c + lane);
     });
     static_for<0, G>([&](auto T) { write_at(T); });
     lds_fence();
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(256) void k_sweep_seg_lds(const double *x, double *
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         const int idx = c * 64 + l64;
-        chunk[c] = src[idx < WE ? idx : WE - 1];
+        chunk[c] = ld_stream(src + (idx < WE ? idx : WE - 1));
     }
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
@@ -1448,14 +1448,14 @@ int hs_l1gs_set_variant(hs_l1gs_t h, int variant) {
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr char kMagic[8] = {'H', 'S', 'L', '1', 'G', 'S', '0', '1'};
+constexpr char kMagic[8] = {'H', 'S', 'L', '1', 'G', 'S', '0', '2'};
 
 struct FileHeader {
     char magic[8];
     int64_t n, P, nparts, nchunks, chunksize, S;
     int32_t sweep, cache, prec, fmt, sym, rowwise, variant, pad;
     int64_t nbytes[7];  // D, Df, T, vptr, vind, vdiag, vval
-    uint64_t checksum;  // FNV-1a 64 of the payload
+    uint64_t checksum;  // FNV-1a 64 of the header before this field, then of the payload
 };
 
 uint64_t fnv1a(uint64_t h, const uint8_t *p, size_t n) {
@@ -1464,6 +1464,76 @@ uint64_t fnv1a(uint64_t h, const uint8_t *p, size_t n) {
         h *= 0x100000001b3ULL;
     }
     return h;
+}
+
+uint64_t header_hash(const FileHeader &hd) {
+    return fnv1a(0xcbf29ce484222325ULL, reinterpret_cast<const uint8_t *>(&hd),
+                 offsetof(FileHeader, checksum));
+}
+
+// Largest row count and packed-buffer size a file may describe: n * 8 and the
+// buffer byte counts stay far from int64 overflow.
+constexpr int64_t kMaxRows = int64_t(1) << 40;
+constexpr int64_t kMaxElems = int64_t(1) << 50;
+
+// Everything the loaded plan's kernels index by is recomputed from (n, P, sweep,
+// cache, prec, fmt, sym) exactly as hs_l1gs_build derives it, and the header
+// must agree with it: the partitioning, S, the layout switches and every array
+// size.  Returns an empty string or what is wrong.
+std::string check_header(const FileHeader &hd) {
+    const int64_t n = hd.n, P = hd.P;
+    if (n < 0 || n > kMaxRows || P <= 0 || P > kMaxRows) return "rows or partsize out of range";
+    if (hd.sweep < 0 || hd.sweep > 2) return "unknown sweep";
+    if (hd.cache != HS_CACHE_PACKED && hd.cache != HS_CACHE_VIEW && hd.cache != HS_CACHE_PACKED_SYM)
+        return "unknown cache";
+    if (hd.prec < HS_PREC_FP64 || hd.prec > HS_PREC_BF16) return "unknown precision";
+    if (hd.fmt != HS_FMT_CSC && hd.fmt != HS_FMT_CSR) return "unknown matrix format";
+    if (hd.sym != 0 && hd.sym != 1) return "bad isSymA flag";
+    if (hd.rowwise != ((hd.fmt == HS_FMT_CSR || hd.sym) ? 1 : 0)) return "bad row-wise flag";
+    if (hd.variant < 0 || hd.variant > 2 || hd.pad != 0) return "bad kernel variant";
+    if (hd.cache == HS_CACHE_PACKED_SYM &&
+        !(hd.sweep == HS_SWEEP_SYMMETRIC && hd.sym == 1 && P <= 64))
+        return "one-triangle cache without isSymA, SymmetricSweep and partsize <= 64";
+    if (hd.cache == HS_CACHE_VIEW && hd.prec != HS_PREC_FP64)
+        return "MatrixViewCache in a precision other than FP64";
+    const int64_t nparts = n > 0 ? (n + P - 1) / P : 0;
+    if (hd.nparts != nparts) return "partition count";
+    if (hd.nchunks < 1 || hd.chunksize < 0) return "chunking";
+    if (nparts > 0 && (hd.chunksize < 1 || hd.chunksize > (nparts + hd.nchunks - 1) / hd.nchunks))
+        return "chunking";
+    if (nparts == 0 && hd.chunksize != 0) return "chunking";
+    // S = P(P-1)(nparts-1)/2 + Pl(Pl-1)/2, Pl the last partition's size (:699-701)
+    __int128 S = 0;
+    if (n > 0) {
+        const int64_t last = n - (nparts - 1) * P;
+        S = (__int128)P * (P - 1) * (nparts - 1) / 2 + (__int128)last * (last - 1) / 2;
+    }
+    if (S > kMaxElems || hd.S != (int64_t)S) return "packed size";
+    const int64_t elem = hd.prec == HS_PREC_FP64 ? 8 : (hd.prec == HS_PREC_FP32 ? 4 : 2);
+    const int64_t tri = hd.cache == HS_CACHE_VIEW
+                            ? 0
+                            : (hd.cache == HS_CACHE_PACKED && hd.sweep == HS_SWEEP_SYMMETRIC
+                                   ? 2 * hd.S
+                                   : hd.S);
+    if (hd.nbytes[0] != n * 8 || hd.nbytes[1] != (hd.prec == HS_PREC_FP64 ? 0 : n * 4) ||
+        hd.nbytes[2] != tri * elem)
+        return "array sizes";
+    if (hd.cache != HS_CACHE_VIEW) {
+        for (int k = 3; k < 7; ++k)
+            if (hd.nbytes[k] != 0) return "array sizes";
+        return "";
+    }
+    // MatrixViewCache: the matrix as given (1-based pointers, indices, values)
+    // and, for a non-symmetric CSC, its diagonal positions
+    const int64_t nnzb = hd.nbytes[4];
+    if (n > 0 && (nnzb < 0 || nnzb % 8 != 0 || nnzb / 8 > kMaxElems || hd.nbytes[6] != nnzb ||
+                  hd.nbytes[3] != (n + 1) * 8 || hd.nbytes[5] != (hd.rowwise ? 0 : n * 8)))
+        return "matrix sizes";
+    if (n == 0) {
+        for (int k = 3; k < 7; ++k)
+            if (hd.nbytes[k] != 0) return "matrix sizes";
+    }
+    return "";
 }
 
 struct FileCloser {
@@ -1497,9 +1567,9 @@ int hs_l1gs_save(hs_l1gs_t h, const char *path) {
         const size_t sizes[7] = {h->D.bytes(), h->Df.bytes(), h->T.bytes(), h->vptr.bytes(),
                                  h->vind.bytes(), h->vdiag.bytes(), h->vval.bytes()};
         std::vector<std::vector<uint8_t>> host(7);
-        uint64_t ck = 0xcbf29ce484222325ULL;
+        for (int k = 0; k < 7; ++k) hd.nbytes[k] = (int64_t)sizes[k];
+        uint64_t ck = header_hash(hd);
         for (int k = 0; k < 7; ++k) {
-            hd.nbytes[k] = (int64_t)sizes[k];
             host[k].resize(sizes[k]);
             if (sizes[k])
                 HS_CHECK_HIP(hipMemcpy(host[k].data(), dev[k], sizes[k], hipMemcpyDeviceToHost));
@@ -1528,28 +1598,10 @@ int hs_l1gs_load(const char *path, void *stream, hs_l1gs_t *out) {
         HS_REQUIRE(std::memcmp(hd.magic, kMagic, 8) == 0, HS_EINVAL,
                    "not a saved L1GS preconditioner (bad magic)");
         const int64_t n = hd.n;
-        HS_REQUIRE(n >= 0 && hd.P > 0 && hd.sweep >= 0 && hd.sweep <= 2 &&
-                       (hd.cache == HS_CACHE_PACKED || hd.cache == HS_CACHE_VIEW ||
-                        (hd.cache == HS_CACHE_PACKED_SYM && hd.sweep == HS_SWEEP_SYMMETRIC &&
-                         hd.P <= 64)) &&
-                       hd.prec >= HS_PREC_FP64 && hd.prec <= HS_PREC_BF16 && hd.variant >= 0 &&
-                       hd.variant <= 2,
-                   HS_EINVAL, "corrupt L1GS file (header fields)");
-        const int64_t nparts = n > 0 ? (n + hd.P - 1) / hd.P : 0;
-        const int64_t elem = hd.prec == HS_PREC_FP64 ? 8 : (hd.prec == HS_PREC_FP32 ? 4 : 2);
-        const int64_t tri_elems =
-            hd.cache == HS_CACHE_VIEW
-                ? 0
-                : (hd.cache == HS_CACHE_PACKED && hd.sweep == HS_SWEEP_SYMMETRIC ? 2 * hd.S : hd.S);
-        HS_REQUIRE(hd.nparts == nparts && hd.nbytes[0] == n * 8 &&
-                       hd.nbytes[1] == (hd.prec == HS_PREC_FP64 ? 0 : n * 4) &&
-                       hd.nbytes[2] == tri_elems * elem,
-                   HS_EINVAL, "corrupt L1GS file (array sizes)");
-        for (int k = 3; k < 7; ++k)
-            HS_REQUIRE(hd.nbytes[k] >= 0 && hd.nbytes[k] % 8 == 0, HS_EINVAL,
-                       "corrupt L1GS file (array sizes)");
+        const std::string bad = check_header(hd);
+        HS_REQUIRE(bad.empty(), HS_EINVAL, "corrupt L1GS file (" + bad + ")");
         std::vector<std::vector<uint8_t>> host(7);
-        uint64_t ck = 0xcbf29ce484222325ULL;
+        uint64_t ck = header_hash(hd);
         for (int k = 0; k < 7; ++k) {
             host[k].resize((size_t)hd.nbytes[k]);
             HS_REQUIRE(hd.nbytes[k] == 0 ||
@@ -1590,9 +1642,6 @@ int hs_l1gs_load(const char *path, void *stream, hs_l1gs_t *out) {
         if (h->cache == HS_CACHE_VIEW && n > 0) {
             // the view kernels index through the stored matrix
             const int64_t nnz = (int64_t)(hd.nbytes[4] / 8);
-            HS_REQUIRE(hd.nbytes[3] == (n + 1) * 8 && hd.nbytes[6] == nnz * 8 &&
-                           (h->rowwise || hd.nbytes[5] == n * 8),
-                       HS_EINVAL, "corrupt L1GS file (matrix sizes)");
             validate_sparse_device(n, nnz, h->vptr.get(), h->vind.get(), st);
             if (!h->rowwise) {
                 // diagonal positions recomputed from the validated matrix rather

@@ -57,6 +57,26 @@ __host__ __device__ __forceinline__ double dl1_rule(double a, double d, double e
 }
 
 
+// Aligned full-wave chunk loads of the packed triangles, which an apply
+// streams once from HBM: nontemporal (the `nt` cache policy).  A plain read
+// of 32 KB blocks, one wave each, runs at 5.7 TB/s on this chip with ordinary
+// loads and at 6.2 TB/s with nontemporal ones.  Only for loads that touch
+// each line once: the column-by-column loads of the direct kernels overlap
+// the previous column's lines, which an `nt` load does not keep in cache
+// (measured: those kernels slow down, up to 2x).
+template <class TS>
+__device__ __forceinline__ TS ld_stream(const TS *p) {
+    if constexpr (std::is_same_v<TS, double> || std::is_same_v<TS, float>) {
+        return __builtin_nontemporal_load(p);
+    } else {
+        static_assert(sizeof(TS) == 2, "16-bit storage type");
+        const unsigned short b = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+        TS r;
+        __builtin_memcpy(&r, &b, 2);
+        return r;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // sweep kernels
 // ---------------------------------------------------------------------------
@@ -182,6 +202,10 @@ __device__ __forceinline__ TC bcast_step(TC v, TC &keep, int base4, int lane) {
     if constexpr (PM == 64) return bcast_keep<64>(v, J, keep);
     else return bcast_keep_perm<J>(v, keep, base4, lane);
 }
+// index of the calling lane's PM-lane segment within its wave
+template <int PM>
+__device__ __forceinline__ int seg_of_wave() { return (int)((threadIdx.x & 63) / PM); }
+
 // base4 of the calling lane, opaque to the optimiser: per-step addresses
 // base4 + 4j must fold into the instruction offset, not be precomputed.
 __device__ __forceinline__ int seg_base4(int PM) {
@@ -512,6 +536,111 @@ __device__ __forceinline__ TC sweep_full_symt64(TC xr, TC dr, TC rr, int lane,
 
 
 
+// Staged form of the one-triangle symmetric sweep, for a wave whose 64 / PM
+// segments hold consecutive full partitions: their lower-triangle blocks are
+// one contiguous run of WE = 32 (PM - 1) values in the packed buffer.  The
+// wave copies that run into LDS unchanged, with aligned full-wave nontemporal
+// loads (lane l of chunk c takes value 64 c + l), and both passes read the
+// copy: forward column j of a segment's block is contiguous (one value per
+// lane), and the backward pass reads U(i, j) = L(j, i) at colL(i) + (j - i - 1)
+// -- one fixed offset per lane plus the step, so an immediate offset per step.
+// All loads are issued first; each chunk is written to LDS just before the
+// first forward read that needs it, so the first steps start while later
+// chunks are still in flight.  Lanes with no term at a step read an in-block
+// value into an accumulator that is no longer used (as in sweep_full).  The
+// loads, products and their order are those of sweep_full_symt, so the
+// results are bit-identical.  `wl`: the wave's LDS area (>= NCH * 64 values),
+// `wsrc`: the packed block of the wave's first partition.
+template <int PM>
+struct SymtStage {
+    static constexpr int SEGS = 64 / PM;
+    static constexpr int BLK = PM * (PM - 1) / 2;
+    static constexpr int WE = SEGS * BLK;
+    static constexpr int NCH = (WE + 63) / 64;
+    static constexpr int colL(int j) { return j * (PM - 1) - j * (j - 1) / 2; }
+    // chunks that must be in LDS before forward column j is read (all segments)
+    static constexpr int need(int j) {
+        const int hi = (SEGS - 1) * BLK + colL(j + 1) - 1;  // highest value read
+        return hi / 64 + 1;
+    }
+};
+
+template <int PM, class TS, class TC>
+__device__ __forceinline__ TC sweep_symt_staged(TC xr, TC dr, TC rr, int lane, int seg,
+                                                const TS *__restrict__ wsrc, TS *wl) {
+    using St = SymtStage<PM>;
+    constexpr int NL = PM - 1, NCH = St::NCH, WE = St::WE, BLK = St::BLK;
+    constexpr int RB = NL < 4 ? NL : 4;  // reads in flight, at most one per column
+    static_assert(St::need(NL - 1) == NCH, "the forward pass reads every chunk");
+    const int l64 = (int)(threadIdx.x & 63);
+    TS chunk[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int idx = 64 * c + l64;
+        chunk[c] = ld_stream(wsrc + (idx < WE ? idx : WE - 1));
+    }
+    auto lds_fence = [] { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); };
+    // chunks [A, B) to LDS (compile-time range)
+    auto write_chunks = [&](auto A, auto B) {
+        static_for<decltype(A)::value, decltype(B)::value>([&](auto C) {
+            constexpr int c = decltype(C)::value;
+            if (64 * c + l64 < WE) wl[64 * c + l64] = chunk[c];
+        });
+    };
+    // per-lane bases, opaque so that per-step addresses fold into immediate offsets
+    int fb = seg * BLK + lane - 1;  // forward column j: fb + colL(j) - j (j >= 1)
+    int bb = seg * BLK + St::colL(lane) - lane - 1;  // backward row j: bb + j
+    asm volatile("" : "+v"(fb), "+v"(bb));
+    const int f0 = seg * BLK + (lane > 0 ? lane - 1 : 0);  // forward column 0
+    auto fwd_read = [&](auto J) -> TS {
+        constexpr int j = decltype(J)::value;
+        if constexpr (j == 0) return wl[f0];
+        else return wl[fb + (St::colL(j) - j)];
+    };
+    const int base4 = seg_base4(PM);
+
+    write_chunks(std::integral_constant<int, 0>{}, std::integral_constant<int, St::need(RB - 1)>{});
+    lds_fence();
+    TS rb[RB];
+    static_for<0, RB>([&](auto C) { rb[decltype(C)::value] = fwd_read(C); });
+    RowAcc<TC> row(xr);
+    TC yfin = (TC)0;
+    static_for<0, NL>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        __builtin_amdgcn_sched_barrier(0);
+        const TS v = rb[j % RB];
+        if constexpr (j + RB < NL) {
+            constexpr int a = St::need(j + RB - 1), b = St::need(j + RB);
+            if constexpr (b > a) {
+                write_chunks(std::integral_constant<int, a>{}, std::integral_constant<int, b>{});
+                lds_fence();
+            }
+            rb[j % RB] = fwd_read(std::integral_constant<int, j + RB>{});
+        }
+        TC q = row.q(dr, rr);
+        TC yj = bcast_step<PM, j>(q, yfin, base4, lane);
+        row.add(load_as<TC>(v), yj);  // dead for lanes <= j
+    });
+    TC q = row.q(dr, rr);
+    TC yr = (lane == PM - 1) ? q : yfin;
+    const TC xb = dr * yr;  // y .= D_Dl1 .* y  (:897)
+    static_for<0, RB>([&](auto C) { rb[decltype(C)::value] = wl[bb + (NL - decltype(C)::value)]; });
+    row = RowAcc<TC>(xb);
+    yfin = (TC)0;
+    static_for<0, NL>([&](auto JJ) {
+        constexpr int jj = decltype(JJ)::value;
+        constexpr int j = PM - 1 - jj;
+        __builtin_amdgcn_sched_barrier(0);
+        const TS v = rb[jj % RB];
+        if constexpr (jj + RB < NL) rb[jj % RB] = wl[bb + (NL - (jj + RB))];
+        TC qb = row.q(dr, rr);
+        TC yj = bcast_step<PM, j>(qb, yfin, base4, lane);
+        row.add(load_as<TC>(v), yj);  // dead for lanes >= j
+    });
+    q = row.q(dr, rr);
+    return (lane == 0) ? q : yfin;
+}
+
 // One partition per PM-lane segment; returns x_i * y_i of the lane's row
 // (r_i * z_i in the CG forms) or 0.  `input(i)` supplies the sweep's input
 // for row i (and may update vectors of that row).  Every lane of the wave
@@ -553,6 +682,16 @@ __device__ __forceinline__ double sweep_seg_run_io(In &&input, Out &&output,
     TC yr;
     if (__any(!safe_divisor(dr))) {
         yr = sweep_generic<PM, MODE, false, TS, TC>(xr, dr, rr, lane, Pa, live, Lk, Uk);
+    } else if (MODE == MODE_SYMT && P == PM &&
+               __all(valid && (PAD || Pr == PM) && (k % SymtStage<PM>::SEGS) == seg_of_wave<PM>())) {
+        // every segment of the wave holds a full partition, the wave's
+        // partitions consecutive: stage their blocks (LDS areas of a wave's
+        // segments are contiguous, symt_lds(PM) values each)
+        if constexpr (MODE == MODE_SYMT) {
+            const int sw = seg_of_wave<PM>();
+            yr = sweep_symt_staged<PM, TS, TC>(xr, dr, rr, lane, sw, T + (kk - sw) * SymtStage<PM>::BLK,
+                                               lds - sw * symt_lds(PM));
+        }
     } else if (__all(Pa == PM)) {
         if constexpr (MODE == MODE_SYMT && PM == 64)
             yr = sweep_full_symt64<32, TS, TC>(xr, dr, rr, lane, Lk, lds);

@@ -186,26 +186,93 @@ def test_problem_generators():
     assert min(sizes) >= 700 and max(sizes) <= 3.0e5
 
 
+def _fnv1a(data, h=0xcbf29ce484222325):
+    for b in data:
+        h = ((h ^ b) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+_HDR = "<8s6q8i7q"  # the saved preconditioner's header before its checksum
+
+
+def _saved_file(path, n=100, P=10, nparts=10, nchunks=1, chunksize=10, S=None, sweep=2, cache=0,
+                prec=0, fmt=0, sym=1, rowwise=1, variant=2, nbytes=None, payload=None,
+                checksum=None):
+    """A file in the layout hs_l1gs_save writes (header, FNV-1a checksum of the
+    header and payload), with every field settable, for the loader's checks."""
+    import struct
+    if S is None:
+        last = n - (nparts - 1) * P
+        S = P * (P - 1) * (nparts - 1) // 2 + last * (last - 1) // 2
+    if nbytes is None:
+        tri = 2 * S if sweep == 2 and cache == 0 else S
+        nbytes = [n * 8, 0, tri * 8, 0, 0, 0, 0]
+    if payload is None:
+        payload = bytes(sum(nbytes))
+    head = struct.pack(_HDR, b"HSL1GS02", n, P, nparts, nchunks, chunksize, S, sweep, cache,
+                       prec, fmt, sym, rowwise, variant, 0, *nbytes)
+    ck = _fnv1a(payload, _fnv1a(head)) if checksum is None else checksum
+    path.write_bytes(head + struct.pack("<Q", ck) + payload)
+    return path
+
+
 def test_l1gs_load_rejects_bad_files(tmp_path):
     """hs_l1gs_load checks the file before any device work: missing file,
-    foreign magic, short header and impossible sizes are HS_EINVAL."""
-    import ctypes as C
-    import struct
+    foreign magic, short header, a header that disagrees with the partitioning
+    hs_l1gs_build derives from (n, partsize) -- forged packed size S, row-wise
+    flag, partition count, chunking, array sizes -- and a checksum that does
+    not cover the header are all HS_EINVAL."""
     import hybridsmoothers._lib as L
     h = C.c_void_p()
-    assert L.lib.hs_l1gs_load(str(tmp_path / "missing.bin").encode(), None, C.byref(h)) == L.HS_EINVAL
+
+    def load(path):
+        h.value = None
+        st = L.lib.hs_l1gs_load(str(path).encode(), None, C.byref(h))
+        return st, L.lib.hs_last_error().decode()
+
+    assert load(tmp_path / "missing.bin")[0] == L.HS_EINVAL
     bad = tmp_path / "bad.bin"
     bad.write_bytes(b"NOTL1GS0" + bytes(200))
-    assert L.lib.hs_l1gs_load(str(bad).encode(), None, C.byref(h)) == L.HS_EINVAL
-    assert b"magic" in L.lib.hs_last_error()
+    st, err = load(bad)
+    assert st == L.HS_EINVAL and "magic" in err
+    old = tmp_path / "old.bin"  # the previous format, whose checksum skipped the header
+    old.write_bytes(b"HSL1GS01" + bytes(200))
+    st, err = load(old)
+    assert st == L.HS_EINVAL and "magic" in err
     short = tmp_path / "short.bin"
-    short.write_bytes(b"HSL1GS01" + bytes(10))
-    assert L.lib.hs_l1gs_load(str(short).encode(), None, C.byref(h)) == L.HS_EINVAL
-    # a header whose array sizes do not match its partitioning
-    hdr = struct.pack("<8s6q8i7qQ", b"HSL1GS01", 100, 10, 10, 1, 10, 450,
-                      2, 0, 0, 0, 1, 1, 2, 0, 800, 0, 999, 0, 0, 0, 0, 0)
-    wrong = tmp_path / "wrong.bin"
-    wrong.write_bytes(hdr)
-    assert L.lib.hs_l1gs_load(str(wrong).encode(), None, C.byref(h)) == L.HS_EINVAL
-    assert b"sizes" in L.lib.hs_last_error()
-    assert not h.value
+    short.write_bytes(b"HSL1GS02" + bytes(10))
+    assert load(short)[0] == L.HS_EINVAL
+    # well-formed apart from one field (checksums recomputed, so only the
+    # header check can refuse them)
+    forged = {
+        # S too small for (n, P), with the triangle bytes to match: the packed
+        # sweeps would index past the buffer
+        "packed size": dict(S=300, nbytes=[800, 0, 2 * 300 * 8, 0, 0, 0, 0]),
+        # a CSC without isSymA read as rows: skips the diagonal-position checks
+        "row-wise flag": dict(sym=0, rowwise=1),
+        "row-wise flag ": dict(fmt=1, sym=0, rowwise=0),
+        "partition count": dict(nparts=9),
+        "chunking": dict(nchunks=2, chunksize=6),
+        "array sizes": dict(nbytes=[800, 0, 2 * 450 * 8, 8, 0, 0, 0]),
+        "rows or partsize out of range": dict(n=1 << 41, nparts=(1 << 41) // 10,
+                                              nbytes=[0] * 7),
+        "unknown matrix format": dict(fmt=3),
+        "one-triangle cache": dict(cache=2, sym=0, rowwise=0),
+        "matrix sizes": dict(cache=1, nbytes=[800, 0, 0, 101 * 8, 24, 0, 16]),
+    }
+    for what, kw in forged.items():
+        st, err = load(_saved_file(tmp_path / "forged.bin", **kw))
+        assert st == L.HS_EINVAL, what
+        assert what.strip() in err, (what, err)
+        assert not h.value
+    # a consistent header with a wrong checksum
+    g = _saved_file(tmp_path / "ck2.bin", checksum=12345)
+    st, err = load(g)
+    assert st == L.HS_EINVAL and "checksum" in err
+    # a header field changed after saving (variant 2 -> 1, both valid): only
+    # the header's share of the checksum catches it
+    raw = bytearray(_saved_file(tmp_path / "ck3.bin").read_bytes())
+    raw[8 + 6 * 8 + 6 * 4] = 1
+    (tmp_path / "ck3.bin").write_bytes(bytes(raw))
+    st, err = load(tmp_path / "ck3.bin")
+    assert st == L.HS_EINVAL and "checksum" in err

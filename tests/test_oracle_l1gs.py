@@ -280,3 +280,33 @@ def test_symmetric_wrapper_matches_full():
         half = sp.triu(A) if uplo == "U" else sp.tril(A)
         full = Symmetric(half.tocsc(), uplo).full()
         np.testing.assert_array_equal(full.toarray(), A.toarray())
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("partsize", [1, 4, 7, 16, 53])
+def test_dense_bruteforce_nonsym_csr(seed, partsize):
+    """SparseMatrixCSR input is read by rows whatever isSymA says
+    (l1_gauss_seidel.jl:1231-1242, 1351-1374, 1496-1519, 1588-1597): on a
+    non-symmetric matrix the result is the dense block inverse of A itself,
+    not of A^T -- which a row/column swap on the CSR path would produce."""
+    n = 53
+    A = _rand_nonsym(n, 40 + seed)
+    assert abs(A - A.T).max() > 0.1
+    x = np.random.default_rng(200 + seed).uniform(-1, 1, n)
+    R = A.tocsr()
+    for sweep in ("forward", "backward", "symmetric"):
+        D, Minv = _dense_reference(A, partsize, 1.5, sweep, sym=False)
+        Dt, Minvt = _dense_reference(A, partsize, 1.5, sweep, sym=True)  # A^T
+        for is_sym in (False, True):
+            for cache in ("packed", "view"):
+                p = O.l1gs_build(R, partsize, is_sym=is_sym, sweep=sweep, cache=cache, chunks=3)
+                np.testing.assert_allclose(p.D, D, rtol=1e-14, atol=0)
+                y = O.l1gs_ldiv(p, x)
+                np.testing.assert_allclose(y, Minv @ x, rtol=1e-9, atol=1e-12)
+                if partsize > 1:
+                    assert np.abs(y - Minvt @ x).max() > 1e-6
+        # the CSC of the same matrix (DiagonalIndices path) agrees with the CSR
+        p = O.l1gs_build(A, partsize, sweep=sweep, chunks=2)
+        q = O.l1gs_build(R, partsize, sweep=sweep, chunks=2)
+        np.testing.assert_array_equal(p.D, q.D)
+        np.testing.assert_allclose(O.l1gs_ldiv(p, x), O.l1gs_ldiv(q, x), rtol=1e-13, atol=1e-15)
