@@ -188,3 +188,27 @@ def test_c4_lognormal192_mixed_precision(gpu):
         assert s.status == "solved", key
         assert s.true_relres < 1.3e-8, (key, s.true_relres)
         assert s.iters <= base.iters * 1.01 + 1, (key, s.iters, base.iters)
+
+
+def test_c2_poisson128_cheb4_pcg(gpu):
+    """128^3 Poisson, Chebyshev 4th kind degree 3 as Pl (the 50-step power
+    iteration on the device), CG to rtol 1e-8: the iteration count equals the
+    oracle's PCG run with the device's rho (rho itself agrees with the oracle's
+    power iteration to 1e-13), solutions within 1e-8."""
+    import torch
+    import oracle as O
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.poisson3d(128)
+    n = A.shape[0]
+    b = pb.uniform_rhs(n, 0)
+    C = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend()), A)
+    ref = O.cheb_build(A, 4, 3)
+    assert abs(C.rho - ref.rho) <= 1e-13 * ref.rho
+    ref.rho = C.rho
+    xo, so = O.pcg(A, b, ref, rtol=1e-8)
+    sol = hs.pcg_solve(A, torch.tensor(b, device="cuda"), Pl=C, reltol=1e-8)
+    print(f"[c2 cheb4 d3] iterations {sol.iters} (oracle {so.iters})", flush=True)
+    assert sol.status == "solved"
+    assert sol.iters == so.iters
+    assert relerr(sol.u.cpu().numpy(), xo) < 1e-8
