@@ -106,6 +106,10 @@ uint64_t next_serial();
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Resolves the current device's address of the allocation-free smoother's
+// scratch array (hs_pcg.hip); called by hs_init, once per device.
+void noalloc_scratch_init();
+
 inline void finish(hipStream_t s, int flags) {
     if (!(flags & HS_FLAG_ASYNC)) HS_CHECK_HIP(hipStreamSynchronize(s));
 }

@@ -43,6 +43,8 @@ int hs_init(int device) {
         HS_CHECK_HIP(hipGetDeviceProperties(&prop, device));
         HS_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, HS_ENODEVICE,
                    std::string("libhsmooth is built for gfx950; device is ") + prop.gcnArchName);
+        // this device's address of the allocation-free entry point's scratch
+        hs::noalloc_scratch_init();
     });
 }
 

@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <memory>
+#include <mutex>
 
 namespace hs {
 
@@ -431,6 +432,52 @@ void l1gs_smooth_device(const CSR &A, const L1GSPlan &P, double *x, const double
     }
 }
 
+// ---------------------------------------------------------------------------
+// scratch of the allocation-free smoother round (hs_l1gs_smooth_noalloc): the
+// residual and the correction vector, NOALLOC_ROWS values each, in a
+// file-scope device array.  Each GPU holds its own copy for the life of the
+// process; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (noalloc_scratch), never
+// through the array's name, which in host code is not a device address.
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_ROWS = int64_t(1) << 21;  // 2,097,152 rows: 32 MiB in all
+static __device__ __align__(256) double scratch[2 * NOALLOC_ROWS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// Resolves the calling thread's current device's copy of `scratch`, once per
+// device (the device must be current).
+void noalloc_scratch_init() {
+    int dev = -1;
+    HS_CHECK_HIP(hipGetDevice(&dev));
+    HS_REQUIRE(dev >= 0 && dev < kMaxDevices, HS_EUNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[dev]) return;
+    void *p = nullptr;
+    HS_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    HS_REQUIRE(p != nullptr, HS_EDEVICE, "hipGetSymbolAddress returned no address");
+    g_scratch[dev] = static_cast<double *>(p);
+}
+
+// This device's scratch (resolved here on the device's first use if hs_init
+// was not called for it).
+static double *noalloc_scratch() {
+    int dev = -1;
+    HS_CHECK_HIP(hipGetDevice(&dev));
+    HS_REQUIRE(dev >= 0 && dev < kMaxDevices, HS_EUNSUPPORTED, "device index beyond 63");
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (g_scratch[dev]) return g_scratch[dev];
+    }
+    noalloc_scratch_init();
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[dev];
+}
+
 // The CG workspace of a matrix handle: created on first use, freed with the
 // handle.  Concurrent solves with one matrix handle are not supported.
 static CGWork &work_for(CSR &A) {
@@ -471,6 +518,27 @@ int hs_l1gs_smooth(hs_l1gs_t P, hs_csr_t A, double *x, const double *b, int iter
         l1gs_smooth_device(*A, plan, dx.get(), w.b.get(), iters, w.r.get(), w.z.get(), st);
         HS_CHECK_HIP(hipMemcpyAsync(x, dx.get(), n * 8, hipMemcpyDeviceToHost, st));
         HS_CHECK_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int hs_l1gs_smooth_noalloc(hs_l1gs_t P, hs_csr_t A, double *x, const double *b, int iters,
+                           void *stream, int flags) {
+    return guard([&] {
+        HS_REQUIRE(P && A, HS_EINVAL, "NULL handle");
+        HS_REQUIRE(iters >= 0, HS_EINVAL, "iterations must be non-negative");
+        const int64_t n = A->n;
+        const L1GSPlan &plan = *reinterpret_cast<const L1GSPlan *>(P);
+        HS_REQUIRE(l1gs_rows(plan) == n, HS_EINVAL, "preconditioner size does not match the matrix");
+        HS_REQUIRE(n <= NOALLOC_ROWS, HS_EINVAL,
+                   "system larger than the allocation-free scratch (2,097,152 rows); use "
+                   "hs_l1gs_smooth");
+        HS_REQUIRE(n == 0 || (x && b), HS_EINVAL, "NULL vector");
+        if (n == 0 || iters == 0) return;
+        double *r = noalloc_scratch();
+        // the residual / next iterate and the correction, as hs_l1gs_smooth's
+        // matrix-owned workspace; every value is written before it is read
+        l1gs_smooth_device(*A, plan, x, b, iters, r, r + NOALLOC_ROWS, as_stream(stream));
+        finish(as_stream(stream), flags);
     });
 }
 

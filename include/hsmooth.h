@@ -155,6 +155,20 @@ int hs_cheb_destroy(hs_cheb_t s);
  */
 int hs_l1gs_smooth(hs_l1gs_t P, hs_csr_t A, double *x, const double *b, int iters, int loc,
                    void *stream, int flags);
+/*
+ * No counterpart in the reference.  hs_l1gs_smooth on device vectors that
+ * allocates nothing, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): its residual and correction vectors live in a fixed device
+ * array of the library (2 x 2,097,152 doubles, 32 MiB per GPU, held until
+ * the process ends), so systems of more than 2,097,152 rows are refused with
+ * HS_EINVAL.  x and b are device pointers.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other threads, other streams)
+ * race and must be serialised by the caller.  Same kernels and results as
+ * hs_l1gs_smooth.
+ */
+int hs_l1gs_smooth_noalloc(hs_l1gs_t P, hs_csr_t A, double *x, const double *b, int iters,
+                           void *stream, int flags);
 
 /* ---- preconditioned CG ------------------------------------------------ */
 #define HS_PCG_NONE 0
