@@ -58,6 +58,24 @@ def _compile(src, force):
     return obj, None
 
 
+CHECK_SRC = os.path.join(CSRC, "check", "hs_check.hip")
+CHECK_OUT = os.path.join(HERE, "hybridsmoothers", "libhs_check.so")
+
+
+def build_check(force: bool = False) -> str:
+    """libhs_check.so: host builds of the sweeps' arithmetic helpers for the CPU
+    tests (hs_check.hip), with the library's flags."""
+    deps = _deps(CHECK_SRC)
+    if force or _stale(CHECK_OUT, deps):
+        tmp = CHECK_OUT + ".tmp"
+        cmd = [HIPCC, *FLAGS, "-shared", CHECK_SRC, "-o", tmp]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        shutil.move(tmp, CHECK_OUT)
+    return CHECK_OUT
+
+
 def build(force: bool = False, jobs: int | None = None) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -75,6 +93,7 @@ def build(force: bool = False, jobs: int | None = None) -> str:
         if r.returncode != 0:
             raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
         shutil.move(tmp, OUT)
+    build_check(force)
     return OUT
 
 

@@ -216,14 +216,22 @@ __device__ __forceinline__ int seg_base4(int PM) {
 
 // Correctly rounded z/d from the correctly rounded reciprocal r = 1/d:
 // q0 = z*r, e = z - q0*d (exact with an FMA), q = q0 + e*r rounds to z/d
-// (Markstein's theorem) whenever d, r and z/d are normal numbers -- the
-// sweep checks |d| once per partition and otherwise uses true division.
-__device__ __forceinline__ double quot_fast(double z, double d, double r) {
+// (Markstein's theorem: with r within half an ulp of 1/d and q0 within one
+// ulp of z/d, the correction step gives the correctly rounded quotient,
+// provided nothing overflows or underflows -- e is then exact).  The sweep
+// checks |d| in [2^-400, 2^400] once per partition (else true division); the
+// theorem's remaining conditions hold when z/d is well inside the normal
+// range, |z/d| in [2^-500, 2^500] (then z, z*r and e are normal too).  Outside
+// it the result can differ from z / d: an overflowing quotient gives NaN
+// instead of +-inf, a quotient in the subnormal range can be an ulp off, and a
+// zero remainder z = -0 gives +0.  The same code is compiled for the host
+// (libhs_check.so) and tested there against IEEE division.
+__host__ __device__ __forceinline__ double quot_fast(double z, double d, double r) {
     double q0 = z * r;
     double e = __builtin_fma(-q0, d, z);
     return __builtin_fma(e, r, q0);
 }
-__device__ __forceinline__ float quot_fast(float z, float d, float r) {
+__host__ __device__ __forceinline__ float quot_fast(float z, float d, float r) {
     float q0 = z * r;
     float e = __builtin_fmaf(-q0, d, z);
     return __builtin_fmaf(e, r, q0);
@@ -233,11 +241,11 @@ __device__ __forceinline__ TC quot(TC z, TC d, TC r) {
     if constexpr (FAST) return quot_fast(z, d, r);
     else return z / d;
 }
-__device__ __forceinline__ bool safe_divisor(double d) {
+__host__ __device__ __forceinline__ bool safe_divisor(double d) {
     double a = fabs(d);
     return a >= 0x1p-400 && a <= 0x1p400;
 }
-__device__ __forceinline__ bool safe_divisor(float d) {
+__host__ __device__ __forceinline__ bool safe_divisor(float d) {
     float a = fabsf(d);
     return a >= 0x1p-60f && a <= 0x1p60f;
 }
