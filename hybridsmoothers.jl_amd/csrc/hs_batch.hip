@@ -1382,8 +1382,8 @@ int hs_batch_info(hs_batch_t e, int64_t *out) {
 int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out) {
     return guard([&] {
         HS_REQUIRE(e && out, HS_EINVAL, "NULL argument");
-        HS_REQUIRE(what >= 0 && what <= 8, HS_EINVAL, "unknown quantity");
-        HS_REQUIRE((what != 5 || !e->cheb) && (what < 7 || e->cheb), HS_EINVAL,
+        HS_REQUIRE(what >= 0 && what <= 9, HS_EINVAL, "unknown quantity");
+        HS_REQUIRE((what != 5 || !e->cheb) && (what < 7 || what > 8 || e->cheb), HS_EINVAL,
                    "quantity of the other smoother");
         e->wait_step();
         HS_CHECK_HIP(hipStreamSynchronize(e->us));
@@ -1413,6 +1413,8 @@ int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out) {
             HS_CHECK_HIP(hipMemcpy(out, blk + L.o_part, 2 * L.ntiles * 8, hipMemcpyDeviceToHost));
         } else if (what == 7) {  // l1inv (FP64)
             HS_CHECK_HIP(hipMemcpy(out, blk + L.o_l1, L.n * 8, hipMemcpyDeviceToHost));
+        } else if (what == 9) {  // bytes per stored column as uploaded
+            out[0] = S->c16 ? 2.0 : 4.0;
         } else {  // rho
             out[0] = h.rho;
         }

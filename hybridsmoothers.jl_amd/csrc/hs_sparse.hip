@@ -82,6 +82,13 @@ int hs_csr_info(hs_csr_t A, int64_t *out) {
     });
 }
 
+int hs_csr_column_bytes(hs_csr_t A, int64_t *bytes) {
+    return guard([&] {
+        HS_REQUIRE(A && bytes, HS_EINVAL, "NULL argument");
+        *bytes = A->col16.get() ? 2 : 4;
+    });
+}
+
 int hs_csr_get(hs_csr_t A, int64_t *rowptr, int64_t *colval, double *nzval) {
     return guard([&] {
         HS_REQUIRE(A && rowptr && (A->nnz == 0 || (colval && nzval)), HS_EINVAL, "NULL argument");

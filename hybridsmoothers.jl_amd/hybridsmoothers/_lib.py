@@ -102,6 +102,7 @@ EXPORTED = [
     "hs_l1gs_build", "hs_l1gs_apply", "hs_l1gs_info", "hs_l1gs_get", "hs_l1gs_destroy",
     "hs_l1gs_set_variant", "hs_l1gs_save", "hs_l1gs_load",
     "hs_csr_create", "hs_csr_spmv", "hs_csr_info", "hs_csr_get", "hs_csr_destroy",
+    "hs_csr_column_bytes",
     "hs_cheb_build", "hs_cheb_smooth", "hs_cheb_apply", "hs_cheb_info", "hs_cheb_get",
     "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve", "hs_l1gs_smooth", "hs_l1gs_smooth_noalloc",
     "hs_problem_lognormal",
@@ -117,6 +118,8 @@ def bind_optional():
         _sig("hs_csr_spmv", [vp, vp, vp, C.c_int, vp, C.c_int])
         _sig("hs_csr_info", [vp, vp])
         _sig("hs_csr_get", [vp, vp, vp, vp])
+    if hasattr(lib, "hs_csr_column_bytes"):
+        _sig("hs_csr_column_bytes", [vp, vp])
         _sig("hs_csr_destroy", [vp])
     if hasattr(lib, "hs_cheb_build"):
         _sig("hs_cheb_build", [i64, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp,

@@ -225,13 +225,13 @@ class BatchSolver:
         return dict(zip(keys, (int(t) for t in v)))
 
     _PEEK = {"x": 0, "r": 1, "z": 2, "b": 3, "q": 4, "D": 5, "partials": 6, "l1inv": 7,
-             "rho": 8}
+             "rho": 8, "column_bytes": 9}
 
     def peek(self, sid: int, what: str) -> np.ndarray:
         """A device vector of a resident system after the last step (diagnostics)."""
         x, _ = self._live[sid]
         n = x.size
-        size = {"partials": 2 * ((n + 255) // 256), "rho": 1}.get(what, n)
+        size = {"partials": 2 * ((n + 255) // 256), "rho": 1, "column_bytes": 1}.get(what, n)
         out = np.empty(size, np.float64)
         L.check(L.lib.hs_batch_peek(self._h, sid, self._PEEK[what], out.ctypes.data))
         return out

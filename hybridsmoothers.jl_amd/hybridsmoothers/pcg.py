@@ -37,6 +37,14 @@ class DeviceMatrix:
     def handle(self):
         return self._h
 
+    @property
+    def column_bytes(self) -> int:
+        """Bytes the products read per stored column: 2 (16-bit offsets from the
+        row, every column within 32,767 of it) or 4."""
+        out = C.c_int64(0)
+        L.check(L.lib.hs_csr_column_bytes(self._h, C.byref(out)))
+        return int(out.value)
+
     def mul(self, y, x, stream=None, sync=True):
         py, ly = L.vec_ptr(y, self.n)
         px, lx = L.vec_ptr(x, self.n)

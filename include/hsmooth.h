@@ -124,6 +124,10 @@ int hs_csr_spmv(hs_csr_t A, double *y, const double *x, int loc, void *stream, i
 int hs_csr_info(hs_csr_t A, int64_t *out);
 /* Copies 1-based rowptr (n+1), colval (nnz) and nzval (nnz) to host arrays. */
 int hs_csr_get(hs_csr_t A, int64_t *rowptr, int64_t *colval, double *nzval);
+/* Bytes the products read per stored column: 2 when every column lies within
+ * 32,767 of its row (16-bit offsets), else 4.  No counterpart in the
+ * reference (a layout diagnostic: the operator's algorithmic bytes depend on it). */
+int hs_csr_column_bytes(hs_csr_t A, int64_t *bytes);
 int hs_csr_destroy(hs_csr_t A);
 
 /* ---- Chebyshev smoothers --------------------------------------------- */
@@ -286,7 +290,9 @@ int hs_batch_info(hs_batch_t e, int64_t *out);
  * `out` after the last step: what = 0 x, 1 r, 2 z, 3 b, 4 q (n doubles each),
  * 5 D_Dl1 (n, widened to FP64; L1GS), 6 the two partials of each 256-row
  * tile, 7 l1inv (n, FP64; Chebyshev), 8 rho (one value; Chebyshev, final once
- * the power iteration has ended).  For tests and diagnostics. */
+ * the power iteration has ended), 9 the bytes per stored column of its upload
+ * (one value: 2 for 16-bit offsets from the row, 4 for 32-bit columns).  For
+ * tests and diagnostics. */
 int hs_batch_peek(hs_batch_t e, int64_t id, int what, double *out);
 /* 1 while the last step is still running on the device, 0 once it has
  * finished (or none is pending), negative on a device error.  Lets a caller

@@ -150,3 +150,39 @@ def test_operator_column_formats(gpu, far):
     R.sort_indices()
     np.testing.assert_array_equal(B.indices, R.indices)
     np.testing.assert_array_equal(B.data, R.data)
+
+
+@pytest.mark.parametrize("far", [32767, 32768])
+def test_column_format_boundary(gpu, far):
+    """A coupling exactly 32,767 rows away still fits the 16-bit column offsets
+    (2 bytes per stored column), one 32,768 away does not (4 bytes): on the
+    device-setup path (DeviceMatrix) and on the batched engine's upload, which
+    decide independently, and in the host bytes model the benchmark uses; the
+    products are the same either way."""
+    import oracle as O
+    from hybridsmoothers.batch import BatchSolver, column_bytes
+    hs = gpu
+    n = 40000
+    rng = np.random.default_rng(far)
+    A = sp.diags([rng.uniform(-1, 1, n - 1), rng.uniform(-1, 1, n - far)], [1, far],
+                 shape=(n, n), format="csc")
+    A = (A + A.T + 8.0 * sp.identity(n)).tocsc()
+    A.sort_indices()
+    want = 2 if far <= 32767 else 4
+    assert column_bytes(A.indptr + 1, A.indices + 1) == want
+    M = hs.DeviceMatrix(A)
+    assert M.column_bytes == want
+    x = rng.uniform(-1, 1, n)
+    y = np.empty(n)
+    M.mul(y, x)
+    np.testing.assert_array_equal(y, O.spmv(O.csr1(A), x))
+    b = rng.uniform(-1, 1, n)
+    for kw in (dict(partsize=32), dict(smoother="cheb", degree=2)):
+        with BatchSolver(precision="fp64", passes=4, arena_bytes=64 << 20, max_systems=2,
+                         **kw) as eng:
+            sid = eng.submit(A, b)
+            eng.step()
+            assert int(eng.peek(sid, "column_bytes")[0]) == want
+            while eng.pending():
+                eng.step()
+                eng.collect()
