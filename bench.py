@@ -94,12 +94,13 @@ def time_launches(fn, steps, warmup, stream, flush=None):
             "p90": round(float(np.percentile(t, 90)), 5)}
 
 
-def pcg_bench(hs, M, b, P, label, reps=3):
-    """Time `reps` PCG solves (x0 = 0, rtol 1e-8) after one warm-up solve."""
-    sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8)
+def pcg_bench(hs, M, b, P, label, reps=3, true_reltol=None):
+    """Time `reps` PCG solves (x0 = 0, rtol 1e-8) after one warm-up solve; with
+    true_reltol the solve also continues to that true relative residual."""
+    sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8, true_reltol=true_reltol)
     times = []
     for _ in range(reps):
-        sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8)
+        sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8, true_reltol=true_reltol)
         times.append(sol.seconds)
     t = float(np.median(times))
     return {"name": label, "iters": sol.iters, "status": sol.status,
@@ -107,13 +108,27 @@ def pcg_bench(hs, M, b, P, label, reps=3):
             "solves_per_s": round(1.0 / t, 3), "ms_per_iter": round(t * 1e3 / max(sol.iters, 1), 4)}
 
 
-def cpu_baseline(A, partsize, budget_s=12.0):
-    """The oracle's packed symmetric sweep (KA.CPU mapping, chunks = host cores)."""
+# The CPU baselines use one OpenMP thread per host core up to 16: the GPU
+# hosts give each GPU a share of 16 cores (whatever os.cpu_count() reports
+# there), and more threads than cores would slow the reference's static
+# chunk mapping down, not speed it up.  Both counts go into the record.
+CPU_THREAD_CAP = 16
+
+
+def cpu_threads():
+    return min(os.cpu_count() or 1, CPU_THREAD_CAP)
+
+
+def cpu_baseline(A, partsize, budget_s=12.0, A3=None, c4_rhs=None):
+    """The oracle's packed symmetric sweep (KA.CPU mapping, chunks = host cores),
+    its MatrixViewCache sweep and L1GS PCG at 128^3; the 2048^2 convection-
+    diffusion forward and backward sweeps (P = 32); the Chebyshev 4th kind
+    degree 3 PCG at 128^3."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from hybridsmoothers import problems as pb
     n = A.shape[0]
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     ref = O.l1gs_build(A, partsize, is_sym=True, sweep="symmetric", chunks=threads,
                        nthreads=threads)
     x = pb.uniform_rhs(n, 0)
@@ -129,7 +144,10 @@ def cpu_baseline(A, partsize, budget_s=12.0):
     gbs = sweep_bytes(n, S) / dt / 1e9
     out = {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
            "sample": f"{reps} symmetric L1GS applications of the 128^3 problem (P={partsize}, "
-                     f"FP64 packed), oracle restatement with OpenMP, {dt * 1e3:.1f} ms each"}
+                     f"FP64 packed), oracle restatement with OpenMP, {dt * 1e3:.1f} ms each",
+           "host_cpu_count": os.cpu_count(),
+           "threads_note": f"OpenMP threads = min(os.cpu_count(), {CPU_THREAD_CAP}): the host's "
+                           f"CPU share per GPU is {CPU_THREAD_CAP} cores"}
     # the CPU default layout (MatrixViewCache, :613), a few applications
     view = O.l1gs_build(A, partsize, is_sym=True, sweep="symmetric", cache="view", chunks=threads,
                         nthreads=threads)
@@ -143,6 +161,35 @@ def cpu_baseline(A, partsize, budget_s=12.0):
     _, st = O.pcg(A, x, ref, rtol=1e-8, nthreads=threads)
     out["pcg_ms_per_solve"] = round((time.perf_counter() - t0) * 1e3, 1)
     out["pcg_iters"] = st.iters
+    if c4_rhs is not None:
+        # Chebyshev 4th kind degree 3 as Pl at 128^3 (its own power iteration)
+        t0 = time.perf_counter()
+        c4 = O.cheb_build(A, 4, 3, nthreads=threads)
+        tb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, st4 = O.pcg(A, c4_rhs, c4, rtol=1e-8, nthreads=threads)
+        out["cheb4_d3_pcg"] = {"ms_per_solve": round((time.perf_counter() - t0) * 1e3, 1),
+                               "iters": st4.iters, "build_ms": round(tb * 1e3, 1)}
+    if A3 is not None:
+        # 2048^2 upwind convection-diffusion, P = 32, non-symmetric CSC (DiagonalIndices
+        # path, O(nnz) l1 diagonal), forward and backward, FP64 packed
+        x3 = pb.uniform_rhs(A3.shape[0], 1)
+        c3 = {}
+        for sw in ("forward", "backward"):
+            t0 = time.perf_counter()
+            r3 = O.l1gs_build(A3, 32, is_sym=False, sweep=sw, nthreads=threads)
+            tb = time.perf_counter() - t0
+            O.l1gs_ldiv(r3, x3)
+            reps3, t0 = 0, time.perf_counter()
+            while reps3 < 5 or (time.perf_counter() - t0 < 2.0 and reps3 < 50):
+                O.l1gs_ldiv(r3, x3)
+                reps3 += 1
+            dt3 = (time.perf_counter() - t0) / reps3
+            b3 = r3.SL.size * 8 + 3 * A3.shape[0] * 8 if sw == "forward" else \
+                r3.SU.size * 8 + 3 * A3.shape[0] * 8
+            c3[sw] = {"ms_per_apply": round(dt3 * 1e3, 3), "GB/s": round(b3 / dt3 / 1e9, 2),
+                      "build_ms": round(tb * 1e3, 1), "applications": reps3}
+        out["convdiff2048_p32"] = c3
     return out
 
 
@@ -227,7 +274,8 @@ def mixed_precision_bench(hs, pb, builder, n3=192, log_fn=log):
             P = builder(A, 64, isSymA=True, sweep=hs.SymmetricSweep(), precision=prec,
                         cache_strategy=cache)
             rows.append(pcg_bench(hs, M, b, P,
-                                  f"L1GS P=64 symmetric{tag}, {prec.upper()} smoother", reps=1))
+                                  f"L1GS P=64 symmetric{tag}, {prec.upper()} smoother", reps=1,
+                                  true_reltol=1e-8))
             del P
     base = rows[0]
     for k, r in enumerate(rows):
@@ -236,7 +284,36 @@ def mixed_precision_bench(hs, pb, builder, n3=192, log_fn=log):
         r["speedup_vs_fp64"] = round(r["solves_per_s"] / base["solves_per_s"], 3)
         r["speedup_vs_fp64_same_storage"] = round(r["solves_per_s"] / same["solves_per_s"], 3)
     return {"workload": f"3D 7-point lognormal-coefficient Poisson {n3}^3 (N={n}, "
-                        f"nnz={A.nnz}), seed 2; FP64 CG rtol 1e-8", "solves": rows}
+                        f"nnz={A.nnz}), seed 2; FP64 CG to a true relative residual of 1e-8 "
+                        f"(Krylov.jl's rule at rtol 1e-8, continued on the device while "
+                        f"||b - A x|| / ||b|| > 1e-8)", "solves": rows}
+
+
+def cheb_apply_bench(hs, A, c4, x, y, steps, warmup, stream):
+    """One Chebyshev 4th-kind degree-3 application as a left preconditioner
+    (z = 0; smooth!(z, S, r); x0 = 0, the dead first and last SpMVs skipped) on
+    the 128^3 problem, FP64: three launches (init and two SpMV-fused steps).
+    Algorithmic bytes 2 B_A + 16 N 8 (two products, 16 vector streams), B_A the
+    operator as the products read it: nnz (8-byte value + 2-byte column offset)
+    + 8 (slices + 1); with the Int64 CSR layout the same formula gives
+    768,606,224 B."""
+    n = A.shape[0]
+    M = hs.DeviceMatrix(A)
+    cb = M.column_bytes
+    nslices = (n + 63) // 64
+    b_a = int(A.nnz) * (8 + cb) + (nslices + 1) * 8
+    del M
+    nbytes = 2 * b_a + 16 * n * 8
+    ms = time_kernel(lambda: hs.ldiv_(y, c4, x, sync=False), steps, warmup, stream)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"workload": "3D 7-point Poisson 128^3, ChebyshevFourth degree 3 FP64 as Pl "
+                        "(x0 = 0), one application per step",
+            "ms": round(ms, 5), "algorithmic_bytes": nbytes, "column_bytes": cb,
+            "bytes_int64_csr": 2 * ((n + 1) * 8 + int(A.nnz) * 16) + 16 * n * 8,
+            "GB/s": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "launches": 3,
+            "launch_ms": time_launches(lambda: hs.ldiv_(y, c4, x, sync=False), min(steps, 100),
+                                       warmup, stream)}
 
 
 def amg_smoother_bench(hs, pb, A, builder, steps, warmup):
@@ -310,7 +387,7 @@ def serving_cpu_baseline(budget_s=10.0, smoother="l1gs"):
     import oracle as O
     from hybridsmoothers import problems as pb
     specs = pb.serving_stream(4096, seed=5)
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     done, rows, t0 = 0, 0, time.perf_counter()
     for i, spec in enumerate(specs):
         A = pb.csr_to_scipy(*pb.stream_system(spec))
@@ -508,6 +585,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(int(best)) if g == 128 and P == 64 else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel, scaled "
+                                       "by a plain copy's readings, read from the committed "
+                                       "profiles/l1gs_sym_sweep_fp64_poisson128_p64_traffic.json "
+                                       "(not measured in this run)",
                      "algorithmic_bytes": nbytes},
         "sweeps": variants,
         "kernel_variant_ms": kernel_variants,
@@ -535,6 +616,8 @@ def main():
             del pv
         c4 = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend()), A)
         pcg.append(pcg_bench(hs, M, bvec, c4, "Chebyshev 4th kind degree 3, FP64"))
+        out["chebyshev_apply_128"] = cheb_apply_bench(hs, A, c4, x, y, args.steps, args.warmup,
+                                                      stream)
         c4f = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend(),
                                                     precision="fp32"), A)
         pcg.append(pcg_bench(hs, M, bvec, c4f, "Chebyshev 4th kind degree 3, FP32"))
@@ -560,7 +643,9 @@ def main():
                     smoother="cheb")
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
-        out["cpu_baseline"] = cpu_baseline(A, P)
+        out["cpu_baseline"] = cpu_baseline(
+            A, P, A3=None if args.quick else pb.convdiff2d(args.grid2d),
+            c4_rhs=None if args.quick else pb.uniform_rhs(n, 0))
     print(json.dumps(out), flush=True)
 
 

@@ -307,9 +307,15 @@ static void enqueue_iteration(const CSR &A, const PrecRef &M, CGWork &w, hipStre
 // Solves with x0 = 0; bvec/xvec are device pointers.  Iterations run in
 // blocks of `block` launches between host checks of the done flag; with
 // use_graph the block is captured once into a HIP graph and replayed.
+// true_rtol > 0: at each stop of Krylov.jl's rule the FP64 true relative
+// residual ||b - A x|| / ||b|| is computed on the device, and while it exceeds
+// true_rtol (and iterations remain) the same CG recurrence continues with the
+// stopping threshold eps scaled down by the ratio true_rtol / true residual
+// (times 0.95): no restart, the iterates are those Krylov.jl's rule would
+// produce with a smaller rtol.
 void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, double *xvec,
                 double rtol, double atol, int64_t itmax, int block, bool use_graph,
-                hipStream_t user, hs_pcg_stats *stats) {
+                hipStream_t user, hs_pcg_stats *stats, double true_rtol) {
     const int64_t n = A.n;
     w.ensure(n);
     hipStream_t st = w.stream;
@@ -355,23 +361,56 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
             for (int k = 0; k < 3; ++k) w.graph_key[k] = key[k];
         }
     }
-    for (;;) {
+    // iterations until the device's done flag is set
+    auto run = [&] {
+        for (;;) {
+            HS_CHECK_HIP(
+                hipMemcpyAsync(w.host, w.st.get(), sizeof(CGDev), hipMemcpyDeviceToHost, st));
+            HS_CHECK_HIP(hipStreamSynchronize(st));
+            if (w.host->done) break;
+            if (use_graph) {
+                HS_CHECK_HIP(hipGraphLaunch(w.graph, st));
+            } else {
+                for (int k = 0; k < block; ++k) iteration(st);
+            }
+        }
+    };
+    // true residual ||b - A x||^2, ||b||^2 in FP64 into the device scalars
+    // (and their host mirror)
+    auto true_residual = [&] {
+        hipLaunchKernelGGL(k_cg_resid, dim3(g), dim3(256), 0, st, n, A.view(), w.x.get(), bvec,
+                           w.partials.get(), w.st.get());
+        hipLaunchKernelGGL(k_cg_fin_resid, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), g,
+                           w.st.get());
+        check_launch();
         HS_CHECK_HIP(hipMemcpyAsync(w.host, w.st.get(), sizeof(CGDev), hipMemcpyDeviceToHost, st));
         HS_CHECK_HIP(hipStreamSynchronize(st));
-        if (w.host->done) break;
-        if (use_graph) {
-            HS_CHECK_HIP(hipGraphLaunch(w.graph, st));
-        } else {
-            for (int k = 0; k < block; ++k) iteration(st);
+        const CGDev &h = *w.host;
+        return h.b2 > 0 ? std::sqrt(h.res2 / h.b2) : std::sqrt(h.res2);
+    };
+    run();
+    double relres = true_residual();
+    for (int check = 0; check < 64 && true_rtol > 0.0 && relres > true_rtol; ++check) {
+        CGDev h = *w.host;
+        // only a stop of the rule itself continues (not the iteration limit or a
+        // breakdown, nor an exactly zero residual)
+        if (h.status != 0 || h.iter >= h.itmax || !(h.gnext > 0.0) || !(h.gamma > 0.0)) break;
+        h.eps = h.rnorm * (true_rtol / relres) * 0.95;
+        h.beta = h.gnext / h.gamma;  // the update cg_scalars_next skips when it stops
+        h.gamma = h.gnext;
+        h.done = 0;
+        *w.host = h;
+        HS_CHECK_HIP(hipMemcpyAsync(w.st.get(), w.host, sizeof(CGDev), hipMemcpyHostToDevice, st));
+        if (!fused) {
+            // the unfused form updates p = z + beta p after its stopping test
+            hipLaunchKernelGGL(k_cg_pupd, dim3(g), dim3(256), 0, st, n, w.z.get(), w.p.get(),
+                               w.st.get());
+            check_launch();
         }
+        run();
+        relres = true_residual();
     }
-    // true residual ||b - A x|| / ||b|| in FP64
-    hipLaunchKernelGGL(k_cg_resid, dim3(g), dim3(256), 0, st, n, A.view(), w.x.get(), bvec,
-                       w.partials.get(), w.st.get());
-    hipLaunchKernelGGL(k_cg_fin_resid, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), g, w.st.get());
-    check_launch();
     HS_CHECK_HIP(hipMemcpyAsync(xvec, w.x.get(), n * 8, hipMemcpyDeviceToDevice, st));
-    HS_CHECK_HIP(hipMemcpyAsync(w.host, w.st.get(), sizeof(CGDev), hipMemcpyDeviceToHost, st));
     HS_CHECK_HIP(hipStreamSynchronize(st));
     HS_CHECK_HIP(hipEventRecord(w.ev_out, st));
     HS_CHECK_HIP(hipStreamWaitEvent(user, w.ev_out, 0));
@@ -381,7 +420,7 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
     stats->status = h.status;
     stats->rnorm0 = h.rnorm0;
     stats->rnorm = h.rnorm;
-    stats->true_relres = h.b2 > 0 ? std::sqrt(h.res2 / h.b2) : std::sqrt(h.res2);
+    stats->true_relres = relres;
     stats->seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 
@@ -544,7 +583,14 @@ int hs_l1gs_smooth_noalloc(hs_l1gs_t P, hs_csr_t A, double *x, const double *b, 
 
 int hs_pcg_solve(hs_csr_t A, int prec_kind, void *prec, const double *b, double *x, double rtol,
                  double atol, int64_t itmax, int loc, void *stream, hs_pcg_stats *stats) {
+    return hs_pcg_solve_true(A, prec_kind, prec, b, x, rtol, atol, itmax, 0.0, loc, stream, stats);
+}
+
+int hs_pcg_solve_true(hs_csr_t A, int prec_kind, void *prec, const double *b, double *x,
+                      double rtol, double atol, int64_t itmax, double true_rtol, int loc,
+                      void *stream, hs_pcg_stats *stats) {
     return guard([&] {
+        HS_REQUIRE(true_rtol >= 0.0, HS_EINVAL, "true_rtol must be non-negative");
         HS_REQUIRE(A && b && x && stats, HS_EINVAL, "NULL argument");
         HS_REQUIRE(prec_kind == HS_PCG_NONE || prec, HS_EINVAL, "NULL preconditioner");
         HS_REQUIRE(prec_kind >= HS_PCG_NONE && prec_kind <= HS_PCG_CHEB, HS_EINVAL,
@@ -563,12 +609,13 @@ int hs_pcg_solve(hs_csr_t A, int prec_kind, void *prec, const double *b, double 
         if (itmax < 0) itmax = 2 * n;
         const int block = n < 200000 ? 16 : 4;
         if (loc == HS_LOC_DEVICE) {
-            pcg_device(*A, M, w, b, x, rtol, atol, itmax, block, true, user, stats);
+            pcg_device(*A, M, w, b, x, rtol, atol, itmax, block, true, user, stats, true_rtol);
         } else {
             w.ensure(n);
             HS_CHECK_HIP(hipMemcpyAsync(w.b.get(), b, n * 8, hipMemcpyHostToDevice, user));
             DBuf<double> xd(n);
-            pcg_device(*A, M, w, w.b.get(), xd.get(), rtol, atol, itmax, block, true, user, stats);
+            pcg_device(*A, M, w, w.b.get(), xd.get(), rtol, atol, itmax, block, true, user, stats,
+                       true_rtol);
             HS_CHECK_HIP(hipMemcpyAsync(x, xd.get(), n * 8, hipMemcpyDeviceToHost, user));
             HS_CHECK_HIP(hipStreamSynchronize(user));
         }

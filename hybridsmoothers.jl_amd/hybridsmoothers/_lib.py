@@ -104,7 +104,7 @@ EXPORTED = [
     "hs_csr_create", "hs_csr_spmv", "hs_csr_info", "hs_csr_get", "hs_csr_destroy",
     "hs_csr_column_bytes",
     "hs_cheb_build", "hs_cheb_smooth", "hs_cheb_apply", "hs_cheb_info", "hs_cheb_get",
-    "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve", "hs_l1gs_smooth", "hs_l1gs_smooth_noalloc",
+    "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve", "hs_pcg_solve_true", "hs_l1gs_smooth", "hs_l1gs_smooth_noalloc",
     "hs_problem_lognormal",
     "hs_batch_create", "hs_batch_footprint", "hs_batch_submit", "hs_batch_step",
     "hs_batch_collect", "hs_batch_info", "hs_batch_clock", "hs_batch_peek", "hs_batch_busy", "hs_batch_destroy",
@@ -133,6 +133,9 @@ def bind_optional():
     if hasattr(lib, "hs_pcg_solve"):
         _sig("hs_pcg_solve", [vp, C.c_int, vp, vp, vp, C.c_double, C.c_double, i64, C.c_int, vp,
                               C.POINTER(PCGStats)])
+    if hasattr(lib, "hs_pcg_solve_true"):
+        _sig("hs_pcg_solve_true", [vp, C.c_int, vp, vp, vp, C.c_double, C.c_double, i64,
+                                   C.c_double, C.c_int, vp, C.POINTER(PCGStats)])
     if hasattr(lib, "hs_l1gs_smooth"):
         _sig("hs_l1gs_smooth", [vp, vp, vp, vp, C.c_int, C.c_int, vp, C.c_int])
     if hasattr(lib, "hs_l1gs_smooth_noalloc"):

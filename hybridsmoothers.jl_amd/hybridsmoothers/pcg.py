@@ -86,8 +86,13 @@ _STATUS = {0: "solved", 1: "maximum number of iterations exceeded", 2: "breakdow
 
 
 def pcg_solve(A, b, Pl=None, reltol=1e-8, abstol=0.0, maxiters=None, stream=None,
-              matrix: DeviceMatrix | None = None) -> Solution:
-    """FP64 PCG with x0 = 0.  `b` is a numpy (host) or torch CUDA (device) vector."""
+              matrix: DeviceMatrix | None = None, true_reltol: float | None = None) -> Solution:
+    """FP64 PCG with x0 = 0.  `b` is a numpy (host) or torch CUDA (device) vector.
+
+    true_reltol: also meet ||b - A x|| / ||b|| <= true_reltol -- at each stop of
+    Krylov.jl's rule the true residual is computed on the device and the same CG
+    recurrence continues, with a proportionally tighter threshold, while it is
+    above (hs_pcg_solve_true).  None: Krylov.jl's rule alone."""
     M = matrix if matrix is not None else (A if isinstance(A, DeviceMatrix) else DeviceMatrix(A))
     n = M.n
     if isinstance(Pl, L1GSPreconditioner):
@@ -114,8 +119,12 @@ def pcg_solve(A, b, Pl=None, reltol=1e-8, abstol=0.0, maxiters=None, stream=None
     st = stream if stream is not None else (L.current_stream() if lb == L.LOC_DEVICE else None)
     stats = L.PCGStats()
     itmax = -1 if maxiters is None else int(maxiters)
-    L.check(L.lib.hs_pcg_solve(M.handle, kind, ph, pb, px, float(reltol), float(abstol), itmax,
-                               lb, st, C.byref(stats)))
+    if true_reltol is None:
+        L.check(L.lib.hs_pcg_solve(M.handle, kind, ph, pb, px, float(reltol), float(abstol),
+                                   itmax, lb, st, C.byref(stats)))
+    else:
+        L.check(L.lib.hs_pcg_solve_true(M.handle, kind, ph, pb, px, float(reltol), float(abstol),
+                                        itmax, float(true_reltol), lb, st, C.byref(stats)))
     return Solution(x, int(stats.iters), _STATUS.get(int(stats.status), "unknown"),
                     float(stats.rnorm), float(stats.rnorm0), float(stats.true_relres),
                     float(stats.seconds))

@@ -197,6 +197,18 @@ typedef struct {
 int hs_pcg_solve(hs_csr_t A, int prec_kind, void *prec, const double *b, double *x,
                  double rtol, double atol, int64_t itmax, int loc, void *stream,
                  hs_pcg_stats *stats);
+/*
+ * hs_pcg_solve that also meets a stated true residual (no counterpart in the
+ * reference, whose solve stops on the preconditioned norm alone): at each
+ * stop of the rule above the FP64 true relative residual ||b - A x|| / ||b||
+ * is computed on the device, and while it exceeds true_rtol the same CG
+ * recurrence continues (no restart) with its threshold scaled down by
+ * true_rtol / (true residual) * 0.95, until it is met or itmax is reached.
+ * true_rtol = 0 is hs_pcg_solve.
+ */
+int hs_pcg_solve_true(hs_csr_t A, int prec_kind, void *prec, const double *b, double *x,
+                      double rtol, double atol, int64_t itmax, double true_rtol, int loc,
+                      void *stream, hs_pcg_stats *stats);
 
 /* ---- workload generator -------------------------------------------------- */
 /*

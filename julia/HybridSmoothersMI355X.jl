@@ -497,37 +497,39 @@ prec_args(P::MI355XL1GSPreconditioner) = (PCG_L1GS, getfield(P, :handle))
 prec_args(S::MI355XChebyshevSmoother) = (PCG_CHEB, getfield(S, :handle))
 
 """
-    pcg_solve(M::DeviceMatrix, b; Pl = nothing, reltol = 1e-8, abstol = 0.0, maxiters = -1)
+    pcg_solve(M::DeviceMatrix, b; Pl = nothing, reltol = 1e-8, abstol = 0.0, maxiters = -1,
+              true_reltol = 0.0)
 
 FP64 CG from x0 = 0 with Krylov.jl's stopping rule sqrt(r'z) ≤ abstol + reltol·sqrt(r0'z0),
 the preconditioner applied on the device each iteration; returns `(x, stats)` with the
-FP64 true relative residual in `stats`.
+FP64 true relative residual in `stats`.  `true_reltol > 0`: the recurrence continues past
+the rule's stop until ‖b - A x‖/‖b‖ ≤ true_reltol (checked on the device at each stop).
 """
 function pcg_solve(M::DeviceMatrix, b::Vector{Float64}; Pl = nothing, reltol = 1e-8,
-                   abstol = 0.0, maxiters::Integer = -1)
+                   abstol = 0.0, maxiters::Integer = -1, true_reltol = 0.0)
     kind, ph = prec_args(Pl)
     x = similar(b)
     st = Ref{PCGStats}()
     GC.@preserve b x Pl begin
-        check(ccall((:hs_pcg_solve, libhsmooth), Cint,
+        check(ccall((:hs_pcg_solve_true, libhsmooth), Cint,
             (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Int64,
-             Cint, Ptr{Cvoid}, Ref{PCGStats}),
+             Cdouble, Cint, Ptr{Cvoid}, Ref{PCGStats}),
             M.handle, kind, ph, pointer(b), pointer(x), Float64(reltol), Float64(abstol),
-            Int64(maxiters), LOC_HOST, C_NULL, st))
+            Int64(maxiters), Float64(true_reltol), LOC_HOST, C_NULL, st))
     end
     return x, st[]
 end
 function pcg_solve(M::DeviceMatrix, b::ROCVector{Float64}; Pl = nothing, reltol = 1e-8,
-                   abstol = 0.0, maxiters::Integer = -1)
+                   abstol = 0.0, maxiters::Integer = -1, true_reltol = 0.0)
     kind, ph = prec_args(Pl)
     x = similar(b)
     st = Ref{PCGStats}()
     GC.@preserve b x Pl begin
-        check(ccall((:hs_pcg_solve, libhsmooth), Cint,
+        check(ccall((:hs_pcg_solve_true, libhsmooth), Cint,
             (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble, Int64,
-             Cint, Ptr{Cvoid}, Ref{PCGStats}),
+             Cdouble, Cint, Ptr{Cvoid}, Ref{PCGStats}),
             M.handle, kind, ph, pointer(b), pointer(x), Float64(reltol), Float64(abstol),
-            Int64(maxiters), LOC_DEVICE, stream_handle(), st))
+            Int64(maxiters), Float64(true_reltol), LOC_DEVICE, stream_handle(), st))
     end
     return x, st[]
 end

@@ -187,7 +187,7 @@ def test_binds_the_reference_api():
         "hs_l1gs_save", "hs_l1gs_load", "hs_l1gs_smooth", "hs_l1gs_smooth_noalloc",
         "hs_csr_create", "hs_csr_spmv", "hs_csr_info", "hs_csr_get", "hs_csr_destroy",
         "hs_cheb_build", "hs_cheb_smooth", "hs_cheb_apply", "hs_cheb_info", "hs_cheb_get",
-        "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve",
+        "hs_cheb_matrix", "hs_cheb_destroy", "hs_pcg_solve_true",
         "hs_batch_create", "hs_batch_submit", "hs_batch_step", "hs_batch_collect",
         "hs_batch_info", "hs_batch_destroy",
     }
