@@ -862,6 +862,158 @@ __global__ __launch_bounds__(64) void k_sweep_view_wave(const double *x, double 
     if (live) y[g] = yr;
 }
 
+// MatrixViewCache at partsize 64 from the sliced-ELL copy of the rows (the
+// device operator's layout, hs_sparse.hpp): partition k is slice k, so a
+// full-wave load of the slice's entry k gives lane i the k-th stored entry of
+// its own row (value, and its column as a 16-bit offset from the row when
+// every column fits).  The wave loads its rows' W <= WMAX entries once, maps
+// each in-partition entry to the recurrence step that uses it (forward step j
+// for L(i, j), backward step 126 - j for U(i, j) in the symmetric sweep), and
+// runs the packed kernels' step sequence with each step's coefficient taken
+// from a per-lane ring of NR step slots in LDS: every 8 steps a group of slots
+// that has been read is cleared and the entries of the group two ahead are
+// written into it.  A lane only ever reads and writes its own column of the
+// ring (slot * 64 + lane), so no cross-lane ordering is involved.  Slots
+// without an entry hold zero, so each step adds L(i, j) y_j or 0 * y_j exactly
+// as the packed triangles do: the results equal PackedBufferCache's bit for bit.
+// Matrix bytes read once per application: nnz (8 + 2) + 8 (slices + 1).
+namespace viewring {
+constexpr int G = 8;    // steps per group
+constexpr int NR = 16;  // ring slots (two groups)
+template <int MODE>
+constexpr int nstep() { return MODE == MODE_SYM ? 126 : 63; }
+// column of step t
+template <int MODE>
+__host__ __device__ constexpr int step_col(int t) {
+    return MODE == MODE_FWD ? t : (MODE == MODE_BWD ? 63 - t : (t < 63 ? t : 126 - t));
+}
+}  // namespace viewring
+
+template <int MODE, int WMAX>
+__device__ __forceinline__ double view_ring(double xr, double dr, double rr, int lane, int64_t s,
+                                            const SellView<double> &A, int64_t e0, int w,
+                                            double *ring) {
+    using namespace viewring;
+    constexpr int NS = nstep<MODE>();
+    constexpr int NG = (NS + G - 1) / G;
+    static_assert(NR == 2 * G, "the ring holds two groups");
+    // this lane's entries, their steps (-1: none) and values
+    double v[WMAX];
+    int tk[WMAX];
+#pragma unroll
+    for (int k = 0; k < WMAX; ++k) {
+        v[k] = 0.0;
+        tk[k] = -1;
+        if (k < w) {
+            const int64_t p = e0 + 64 * k + lane;
+            const int64_t c = A.col16 ? s + lane + A.col16[p] : (int64_t)A.col[p];
+            const int j = (int)(c - s);  // column within the partition
+            int t = -1;
+            if (MODE != MODE_BWD && j >= 0 && j < lane) t = j;
+            if (MODE == MODE_SYM && j > lane && j < 64) t = 126 - j;
+            if (MODE == MODE_BWD && j > lane && j < 64) t = 63 - j;
+            tk[k] = t;
+            v[k] = A.val[p];
+        }
+    }
+    double *const mine = ring + lane;  // slot u of this lane: mine[64 u]
+    // groups 0 and 1 into the cleared ring
+#pragma unroll
+    for (int u = 0; u < NR; ++u) mine[64 * u] = 0.0;
+#pragma unroll
+    for (int k = 0; k < WMAX; ++k)
+        if (tk[k] >= 0 && tk[k] < NR) mine[64 * tk[k]] = v[k];
+    double col[2][G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) col[0][u] = mine[64 * u];
+
+    RowAcc<double> row(xr);
+    double yfin = 0.0;
+    static_for<0, NG>([&](auto Gi) {
+        constexpr int g = decltype(Gi)::value;
+        constexpr int t0 = g * G;
+        if constexpr ((g + 2) * G < NS) {
+            // group g's slots were read into registers: clear them and write
+            // the entries of group g + 2 (the same slots)
+#pragma unroll
+            for (int u = 0; u < G; ++u) mine[64 * ((t0 + u) % NR)] = 0.0;
+#pragma unroll
+            for (int k = 0; k < WMAX; ++k)
+                if ((tk[k] >> 3) == g + 2) mine[64 * (tk[k] & (NR - 1))] = v[k];
+        }
+        if constexpr ((g + 1) * G < NS) {
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (t0 + G + u < NS) col[(g + 1) & 1][u] = mine[64 * ((t0 + G + u) % NR)];
+        }
+        static_for<t0, t0 + G>([&](auto T) {
+            constexpr int t = decltype(T)::value;
+            if constexpr (t < NS) {
+                if constexpr (MODE == MODE_SYM && t == 63) {
+                    // last forward row, y .= D_Dl1 .* y (:897), then the backward pass
+                    const double q = row.q(dr, rr);
+                    const double yf = (lane == 63) ? q : yfin;
+                    row = RowAcc<double>(dr * yf);
+                    yfin = 0.0;
+                }
+                constexpr int j = step_col<MODE>(t);
+                const double q = row.q(dr, rr);
+                const double yj = bcast_keep<64>(q, j, yfin);
+                row.add(col[g & 1][t - t0], yj);  // zero, or dead, for lanes without a term
+            }
+        });
+    });
+    const double q = row.q(dr, rr);
+    if constexpr (MODE == MODE_FWD) return (lane == 63) ? q : yfin;
+    else return (lane == 0) ? q : yfin;
+}
+
+// partsize 64: full partitions whose slice is at most WMAX wide and whose
+// diagonals are safe take view_ring; others (the short last partition, wider
+// rows, unsafe diagonals) the general view_wave on the 1-based row arrays.
+template <int MODE, int WMAX>
+__global__ __launch_bounds__(64) void k_sweep_view_ring(const double *x, double *y,
+                                                        SellView<double> A,
+                                                        const int64_t *__restrict__ ptr,
+                                                        const int64_t *__restrict__ ind,
+                                                        const double *__restrict__ val,
+                                                        const double *__restrict__ D, int64_t n,
+                                                        int64_t nparts,
+                                                        const int *__restrict__ done) {
+    if (done && *done) return;
+    __shared__ double ring[viewring::NR * 64];
+    const int64_t k = blockIdx.x;
+    if (k >= nparts) return;
+    const int lane = threadIdx.x;
+    const int64_t s = k * 64;
+    const int Pa = (int)(n - s < 64 ? n - s : 64);
+    const bool live = lane < Pa;
+    const int64_t g = s + lane;
+    const double xr = live ? x[g] : 0.0;
+    const double dr = live ? D[g] : 1.0;
+    const double rr = 1.0 / dr;
+    const int64_t e0 = A.sptr[k];
+    const int w = (int)((A.sptr[k + 1] - e0) >> 6);
+    double yr;
+    if (Pa == 64 && w <= WMAX && !__any(!safe_divisor(dr))) {
+        yr = view_ring<MODE, WMAX>(xr, dr, rr, lane, s, A, e0, w, ring);
+    } else {
+        int64_t plo = 0, pmid = 0, pmid1 = 0, phi = 0;
+        if (live) {
+            const int64_t p0 = ptr[g] - 1, p1 = ptr[g + 1] - 1;
+            plo = row_lower_bound(ind, p0, p1, s);
+            pmid = row_lower_bound(ind, plo, p1, g);
+            pmid1 = (pmid < p1 && ind[pmid] - 1 == g) ? pmid + 1 : pmid;
+            phi = row_lower_bound(ind, pmid1, p1, s + Pa);
+        }
+        if (__any(!safe_divisor(dr)))
+            yr = view_wave<MODE, false>(xr, dr, rr, lane, Pa, s, plo, pmid, pmid1, phi, ind, val);
+        else
+            yr = view_wave<MODE, true>(xr, dr, rr, lane, Pa, s, plo, pmid, pmid1, phi, ind, val);
+    }
+    if (live) y[g] = yr;
+}
+
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
@@ -882,6 +1034,10 @@ struct L1GSPlan {
     DBuf<double> vval;
     DBuf<int64_t> wptr, wind;
     DBuf<double> wval;
+    // MatrixViewCache at partsize 64: the rows as sliced ELL (k_sweep_view_ring)
+    // and the widest slice's entry count
+    CSR vsell;
+    int vsell_w = 0;
     Staging stage;
 
     size_t elem() const { return prec == HS_PREC_FP64 ? 8 : (prec == HS_PREC_FP32 ? 4 : 2); }
@@ -1118,11 +1274,43 @@ unsigned l1gs_apply_fused(const L1GSPlan &h, double *z, hipStream_t st, const CG
     return count;
 }
 
+template <int WMAX>
+static void launch_view_ring_w(const L1GSPlan &h, const double *x, double *y, const int64_t *rp,
+                               const int64_t *ri, const double *rv, hipStream_t st,
+                               const int *done) {
+    const dim3 grid(grid_for(h.nparts, 1)), block(64);
+    const SellView<double> A = h.vsell.view();
+    if (h.sweep == HS_SWEEP_FORWARD)
+        hipLaunchKernelGGL((k_sweep_view_ring<MODE_FWD, WMAX>), grid, block, 0, st, x, y, A, rp,
+                           ri, rv, h.D.get(), h.n, h.nparts, done);
+    else if (h.sweep == HS_SWEEP_BACKWARD)
+        hipLaunchKernelGGL((k_sweep_view_ring<MODE_BWD, WMAX>), grid, block, 0, st, x, y, A, rp,
+                           ri, rv, h.D.get(), h.n, h.nparts, done);
+    else
+        hipLaunchKernelGGL((k_sweep_view_ring<MODE_SYM, WMAX>), grid, block, 0, st, x, y, A, rp,
+                           ri, rv, h.D.get(), h.n, h.nparts, done);
+}
+
+// the register budget follows the widest slice: 8 entries (5- and 7-point
+// stencils), 16, or 32 (wider slices take the general kernel wave by wave)
+static void launch_view_ring(const L1GSPlan &h, const double *x, double *y, const int64_t *rp,
+                             const int64_t *ri, const double *rv, hipStream_t st,
+                             const int *done) {
+    if (h.vsell_w <= 8) launch_view_ring_w<8>(h, x, y, rp, ri, rv, st, done);
+    else if (h.vsell_w <= 16) launch_view_ring_w<16>(h, x, y, rp, ri, rv, st, done);
+    else launch_view_ring_w<32>(h, x, y, rp, ri, rv, st, done);
+}
+
 // Device-side application y = M^-1 x (x, y device pointers; may alias).
 void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_t st,
                        const int *done) {
     if (h.n == 0) return;
-    if (h.cache == HS_CACHE_VIEW && h.P <= 64) {
+    if (h.cache == HS_CACHE_VIEW && h.P == 64 && h.vsell.n == h.n) {
+        const int64_t *rp = h.rowwise ? h.vptr.get() : h.wptr.get();
+        const int64_t *ri = h.rowwise ? h.vind.get() : h.wind.get();
+        const double *rv = h.rowwise ? h.vval.get() : h.wval.get();
+        launch_view_ring(h, x, y, rp, ri, rv, st, done);
+    } else if (h.cache == HS_CACHE_VIEW && h.P <= 64) {
         // rows: the matrix itself when it is read row-wise, else its transpose
         const int64_t *rp = h.rowwise ? h.vptr.get() : h.wptr.get();
         const int64_t *ri = h.rowwise ? h.vind.get() : h.wind.get();
@@ -1187,6 +1375,22 @@ static void nonsym_D(L1GSPlan &h, int64_t nnz, const DBuf<int64_t> &ptr, const D
 
 int64_t l1gs_rows(const L1GSPlan &h) { return h.n; }
 uint64_t l1gs_serial(const L1GSPlan &h) { return h.serial; }
+
+// The sliced-ELL copy of the rows the partsize-64 MatrixViewCache kernel reads
+// (the matrix itself when read row-wise, else its transpose), and its widest
+// slice.
+static void build_view_sell(L1GSPlan &h, int64_t nnz, hipStream_t st) {
+    if (h.P != 64 || h.n == 0) return;
+    const int64_t *rp = h.rowwise ? h.vptr.get() : h.wptr.get();
+    const int64_t *ri = h.rowwise ? h.vind.get() : h.wind.get();
+    const double *rv = h.rowwise ? h.vval.get() : h.wval.get();
+    sell_build_device(h.vsell, h.n, nnz, rp, ri, rv, st);
+    std::vector<int64_t> sp(h.vsell.nslices + 1);
+    HS_CHECK_HIP(hipMemcpy(sp.data(), h.vsell.sptr.get(), sp.size() * 8, hipMemcpyDeviceToHost));
+    int64_t wmax = 0;
+    for (int64_t k = 0; k < h.vsell.nslices; ++k) wmax = std::max(wmax, (sp[k + 1] - sp[k]) / 64);
+    h.vsell_w = (int)std::min<int64_t>(wmax, 1 << 20);
+}
 
 template <class TS>
 static void build_packed(L1GSPlan &h, int64_t nnz, const DBuf<int64_t> &ptr,
@@ -1321,6 +1525,7 @@ int hs_l1gs_build(int64_t n, const int64_t *ptr, const int64_t *ind, const doubl
                 h->vind = std::move(dind);
                 h->vval = std::move(dval);
                 h->vdiag = std::move(ddiag);
+                build_view_sell(*h, nnz, st);
             }
             if (precision != HS_PREC_FP64) {
                 h->Df.alloc(n);
@@ -1660,6 +1865,8 @@ int hs_l1gs_load(const char *path, void *stream, hs_l1gs_t *out) {
                 csc_transpose_device(n, nnz, h->vptr.get(), h->vind.get(), h->vval.get(), h->wptr,
                                      h->wind, h->wval, st);
             }
+            // the partsize-64 kernel's sliced-ELL rows are rebuilt, not stored
+            build_view_sell(*h, nnz, st);
         }
         HS_CHECK_HIP(hipStreamSynchronize(st));  // the host buffers go out of scope
         *out = h.release();
