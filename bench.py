@@ -417,6 +417,9 @@ def main():
     ap.add_argument("--partsize", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline sweep only")
+    ap.add_argument("--sweeps-only", action="store_true",
+                    help="the headline and the other sweep variants, no solves (for counter "
+                         "passes)")
     ap.add_argument("--grid2d", type=int, default=2048)
     ap.add_argument("--serve-count", type=int, default=4096)
     ap.add_argument("--serve-slots", type=int, default=1024)
@@ -598,7 +601,7 @@ def main():
         "operator_build_ms": round(t_op * 1e3, 1),
         "kernel_variant": int(best),
     }
-    if not args.quick:
+    if not args.quick and not args.sweeps_only:
         log("[bench] PCG on 128^3 (rtol 1e-8)")
         M = hs.DeviceMatrix(A)
         bvec = torch.tensor(pb.uniform_rhs(n, 0), device="cuda")
@@ -633,7 +636,7 @@ def main():
         out["amg_smoother_128"] = amg_smoother_bench(hs, pb, A, builder, 50, 5)
         if not args.no_mixed:
             out["mixed_precision_192"] = mixed_precision_bench(hs, pb, builder)
-    if not args.quick and not args.no_serve:
+    if not args.quick and not args.no_serve and not args.sweeps_only:
         out["serving"] = serving_bench(args)
         if not args.no_cpu:
             log("[bench] serving CPU baseline (oracle)")
@@ -641,7 +644,7 @@ def main():
             if "chebyshev_d2" in out["serving"]:
                 out["serving"]["chebyshev_d2"]["cpu_baseline"] = serving_cpu_baseline(
                     smoother="cheb")
-    if not args.no_cpu:
+    if not args.no_cpu and not args.sweeps_only:
         log("[bench] CPU baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(
             A, P, A3=None if args.quick else pb.convdiff2d(args.grid2d),

@@ -374,8 +374,14 @@ __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
         yr = sweep_generic<64, MODE_SYM, true, TS, TC>(xr, dr, rr, lane, Pa, live, T + lb, T + ub);
     }
     if (live) {
-        if constexpr (FORM == FORM_SMOOTH) sf.xout[s + lane] = sf.x[s + lane] - (double)yr;
-        else y[s + lane] = (double)yr;
+        if constexpr (FORM == FORM_SMOOTH) {
+            sf.xout[s + lane] = sf.x[s + lane] - (double)yr;
+        } else if constexpr (FORM == FORM_CG && std::is_same_v<TC, float>) {
+            if (f.z32) f.z32[s + lane] = yr;  // an FP32 smoother's z, kept in FP32
+            else y[s + lane] = (double)yr;
+        } else {
+            y[s + lane] = (double)yr;
+        }
     }
     if constexpr (FORM == FORM_CG) {
         double v = live ? xin * (double)yr : 0.0;  // r'z partial of this partition
@@ -1374,6 +1380,7 @@ static void nonsym_D(L1GSPlan &h, int64_t nnz, const DBuf<int64_t> &ptr, const D
 }
 
 int64_t l1gs_rows(const L1GSPlan &h) { return h.n; }
+int l1gs_precision(const L1GSPlan &h) { return h.prec; }
 uint64_t l1gs_serial(const L1GSPlan &h) { return h.serial; }
 
 // The sliced-ELL copy of the rows the partsize-64 MatrixViewCache kernel reads

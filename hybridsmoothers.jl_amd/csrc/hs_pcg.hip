@@ -88,8 +88,9 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_cg_fin_delta(const double *__rest
 // p_k = z + beta p_{k-1} (p_0 = z) formed on the fly, q = A p_k, p_k'q
 // partials.  p_k goes to pbuf[k & 1] while p_{k-1} is gathered from
 // pbuf[(k + 1) & 1], so no thread reads a value another thread rewrites.
+template <class TZ>
 __global__ __launch_bounds__(256) void k_cg_spmv_fused(int64_t n, SellView<double> A,
-                                                       const double *__restrict__ z,
+                                                       const TZ *__restrict__ z,
                                                        double *__restrict__ pa,
                                                        double *__restrict__ pb,
                                                        double *__restrict__ q, double *partials,
@@ -151,6 +152,11 @@ __global__ void k_cg_pupd(int64_t n, const double *__restrict__ z, double *__res
     if (i < n) p[i] = z[i] + beta * p[i];
 }
 
+__global__ void k_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
 __global__ void k_copy(int64_t n, const double *__restrict__ a, double *__restrict__ b,
                        const CGDev *st) {
     if (st && st->done) return;
@@ -197,6 +203,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void k_cg_fin_resid(const double *__rest
 struct CGWork {
     int64_t n = 0;
     DBuf<double> x, r, z, p, p2, q, b, partials;
+    DBuf<float> z32;  // z of an FP32 or BF16 L1GS smoother in the fused iteration
     DBuf<CGDev> st;
     CGDev *host = nullptr;  // pinned mirror of the scalars
     hipStream_t stream = nullptr;
@@ -216,6 +223,7 @@ struct CGWork {
         if (nn != n) {
             n = nn;
             x.alloc(n); r.alloc(n); z.alloc(n); p.alloc(n); p2.alloc(n); q.alloc(n); b.alloc(n);
+            z32.release();
             // block partials of the 256-row kernels (two values each for the
             // residual) or one per partition of the fused sweep (>= 1 row each)
             partials.alloc(std::max<size_t>(2 * (size_t)grid_for(n, 256), (size_t)n));
@@ -265,18 +273,30 @@ static void apply_prec(const PrecRef &M, const double *r, double *z, int64_t n, 
 // with the CG updates folded in (an L1GS sweep: one launch; a 4th-kind
 // Chebyshev smoother of degree d: d launches), finalize gamma.  See
 // k_cg_spmv_fused and CGFuse.
+// Whether the fused iteration keeps z in FP32 (an FP32 or BF16 L1GS smoother).
+static bool z_in_fp32(const PrecRef &M) {
+    return M.kind == HS_PCG_L1GS &&
+           l1gs_precision(*reinterpret_cast<const L1GSPlan *>(M.plan)) != HS_PREC_FP64;
+}
+
 static void enqueue_iteration_fused(const CSR &A, const PrecRef &M, CGWork &w, hipStream_t st) {
     const int64_t n = w.n;
     const unsigned g = grid_for(n, 256);
     CGDev *d = w.st.get();
-    hipLaunchKernelGGL(k_cg_spmv_fused, dim3(g), dim3(256), 0, st, n, A.view(), w.z.get(),
-                       w.p.get(), w.p2.get(), w.q.get(), w.partials.get(), d);
+    const bool z32 = z_in_fp32(M);
+    if (z32)
+        hipLaunchKernelGGL(k_cg_spmv_fused<float>, dim3(g), dim3(256), 0, st, n, A.view(),
+                           w.z32.get(), w.p.get(), w.p2.get(), w.q.get(), w.partials.get(), d);
+    else
+        hipLaunchKernelGGL(k_cg_spmv_fused<double>, dim3(g), dim3(256), 0, st, n, A.view(),
+                           w.z.get(), w.p.get(), w.p2.get(), w.q.get(), w.partials.get(), d);
     hipLaunchKernelGGL(k_cg_fin_delta, dim3(1), dim3(FIN_BLOCK), 0, st, w.partials.get(), g, d);
     check_launch();
     // the sweep reads p_k from the buffer the SpMV above wrote; a graph
     // replays fixed pointers, so both buffers are passed and the parity of
     // st->iter picks one on the device
     CGFuse f{w.r.get(), w.q.get(), w.p.get(), w.p2.get(), w.x.get(), w.partials.get(), d};
+    if (z32) f.z32 = w.z32.get();
     const unsigned cnt =
         M.kind == HS_PCG_L1GS
             ? l1gs_apply_fused(*reinterpret_cast<const L1GSPlan *>(M.plan), w.z.get(), st, f)
@@ -342,6 +362,12 @@ void pcg_device(const CSR &A, const PrecRef &M, CGWork &w, const double *bvec, d
     const bool fused =
         (M.kind == HS_PCG_L1GS && l1gs_can_fuse(*reinterpret_cast<const L1GSPlan *>(M.plan))) ||
         (M.kind == HS_PCG_CHEB && cheb_can_fuse(*reinterpret_cast<const ChebPlan *>(M.plan)));
+    if (fused && z_in_fp32(M)) {
+        // the fused iterations read z in FP32: z0 (FP32 numbers) narrowed exactly
+        if (w.z32.n < (size_t)n) w.z32.alloc(n);
+        hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(256), 0, st, n, w.z.get(), w.z32.get());
+        check_launch();
+    }
     auto iteration = [&](hipStream_t s) {
         if (fused) enqueue_iteration_fused(A, M, w, s);
         else enqueue_iteration(A, M, w, s);

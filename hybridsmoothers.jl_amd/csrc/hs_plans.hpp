@@ -69,13 +69,13 @@ __device__ __forceinline__ bool cg_scalars_delta(CGDev &st, double delta) {
 // formed on the fly for row i and its columns, q_i = (A p_k)_i in row order.
 // Stores p_k[i] and q[i]; returns p_k[i] * q_i (the p'q partial of the row).
 // The row's entries are taken in batches of 8: all column indices and values
-// of a batch are loaded before any gather (indices past the row's slice width
-// re-read its last entry), so a wave keeps its loads in flight instead of one
-// dependent index -> value chain per entry; the products are still added one
-// by one in row order.
-template <bool C16>
+// of a batch are loaded before any gather (entries past the slice width are
+// not loaded), so a wave keeps its loads in flight instead of one dependent
+// index -> value chain per entry; the products are still added one by one in
+// row order.
+template <bool C16, class TZ>
 __device__ __forceinline__ double cg_spmv_row_t(const SellView<double> &A,
-                                                const double *__restrict__ z,
+                                                const TZ *__restrict__ z,
                                                 const double *__restrict__ pold,
                                                 double *__restrict__ pnew, double *__restrict__ q,
                                                 bool first, double beta, int64_t i) {
@@ -89,14 +89,22 @@ __device__ __forceinline__ double cg_spmv_row_t(const SellView<double> &A,
         double v[B], zc[B], pc[B];
 #pragma unroll
         for (int k = 0; k < B; ++k) {
-            const int64_t p = p0 + 64 * (int64_t)min(k0 + k, w - 1);
-            if constexpr (C16) c[k] = (int32_t)(i + A.col16[p]);
-            else c[k] = A.col[p];
-            v[k] = A.val[p];
+            // past the slice width (the same in the whole wave) no load: the
+            // row's own column and a zero that is never added
+            c[k] = (int32_t)i;
+            v[k] = 0.0;
+            if (k0 + k < w) {
+                const int64_t p = p0 + 64 * (int64_t)(k0 + k);
+                // the matrix is streamed once per product: nontemporal, so it
+                // does not displace the gathered z and p from L2
+                if constexpr (C16) c[k] = (int32_t)(i + __builtin_nontemporal_load(A.col16 + p));
+                else c[k] = __builtin_nontemporal_load(A.col + p);
+                v[k] = __builtin_nontemporal_load(A.val + p);
+            }
         }
 #pragma unroll
         for (int k = 0; k < B; ++k) {
-            zc[k] = z[c[k]];
+            zc[k] = (double)z[c[k]];
             pc[k] = first ? 0.0 : pold[c[k]];
         }
 #pragma unroll
@@ -107,18 +115,22 @@ __device__ __forceinline__ double cg_spmv_row_t(const SellView<double> &A,
             }
         }
     }
-    const double pi = first ? z[i] : z[i] + beta * pold[i];
+    const double zi = (double)z[i];
+    const double pi = first ? zi : zi + beta * pold[i];
     pnew[i] = pi;
     q[i] = acc;
     return pi * acc;
 }
+// z in FP64, or in FP32 when the preconditioner computes in FP32 (its values
+// are then FP32 numbers, and widening them gives the same doubles)
+template <class TZ>
 __device__ __forceinline__ double cg_spmv_row(const SellView<double> &A,
-                                              const double *__restrict__ z,
+                                              const TZ *__restrict__ z,
                                               const double *__restrict__ pold,
                                               double *__restrict__ pnew, double *__restrict__ q,
                                               bool first, double beta, int64_t i) {
-    return A.col16 ? cg_spmv_row_t<true>(A, z, pold, pnew, q, first, beta, i)
-                   : cg_spmv_row_t<false>(A, z, pold, pnew, q, first, beta, i);
+    return A.col16 ? cg_spmv_row_t<true, TZ>(A, z, pold, pnew, q, first, beta, i)
+                   : cg_spmv_row_t<false, TZ>(A, z, pold, pnew, q, first, beta, i);
 }
 
 // CG vector updates fused into a symmetric L1GS sweep: before the sweep each
@@ -134,6 +146,10 @@ struct CGFuse {
     double *x;
     double *partials;
     const CGDev *st;
+    // when set (FP32 and BF16 smoothers, whose results are FP32 numbers) z is
+    // stored here in FP32 instead of the FP64 vector: the same values, 4 bytes
+    // a row less written here and read by the next product
+    float *z32 = nullptr;
 };
 
 // Row prologue of the fused CG form: r -= alpha q; x += alpha p (the k_cg_axpy
@@ -172,6 +188,8 @@ void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_
 bool l1gs_can_fuse(const L1GSPlan &h);
 unsigned l1gs_apply_fused(const L1GSPlan &h, double *z, hipStream_t st, const CGFuse &f);
 int64_t l1gs_rows(const L1GSPlan &h);
+// smoother precision of the plan (HS_PREC_*)
+int l1gs_precision(const L1GSPlan &h);
 uint64_t l1gs_serial(const L1GSPlan &h);
 
 // The 4th-kind smoother (degree >= 2) as the fused CG's preconditioner: z =

@@ -736,8 +736,18 @@ __device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
                                                  const CGFuse &f, const SmoothFuse &sf,
                                                  TS *lds = nullptr) {
     if constexpr (FORM == FORM_CG)
-        return sweep_seg_run<PM, MODE, TS, TC, false>([&](int64_t i) { return cg_prologue(f, i); },
-                                                      y, T, D, n, P, nparts, k, lane, lds);
+        return sweep_seg_run_io<PM, MODE, TS, TC, false>(
+            [&](int64_t i) { return cg_prologue(f, i); },
+            [&](int64_t i, double z) {
+                if constexpr (std::is_same_v<TC, float>) {
+                    if (f.z32) {
+                        f.z32[i] = (float)z;  // z is an FP32 number: exact
+                        return;
+                    }
+                }
+                y[i] = z;
+            },
+            T, D, n, P, nparts, k, lane, lds);
     else if constexpr (FORM == FORM_SMOOTH)
         return sweep_seg_run_io<PM, MODE, TS, TC, false>(
             [&](int64_t i) { return row_dot(sf.A, sf.x, i) - sf.b[i]; },
