@@ -94,18 +94,29 @@ def time_launches(fn, steps, warmup, stream, flush=None):
             "p90": round(float(np.percentile(t, 90)), 5)}
 
 
-def pcg_bench(hs, M, b, P, label, reps=3, true_reltol=None):
-    """Time `reps` PCG solves (x0 = 0, rtol 1e-8) after one warm-up solve; with
-    true_reltol the solve also continues to that true relative residual."""
-    sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8, true_reltol=true_reltol)
-    times = []
-    for _ in range(reps):
-        sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8, true_reltol=true_reltol)
-        times.append(sol.seconds)
-    t = float(np.median(times))
-    return {"name": label, "iters": sol.iters, "status": sol.status,
-            "true_relres": sol.true_relres, "ms_per_solve": round(t * 1e3, 3),
-            "solves_per_s": round(1.0 / t, 3), "ms_per_iter": round(t * 1e3 / max(sol.iters, 1), 4)}
+def pcg_bench_interleaved(hs, M, b, configs, rounds=3, true_reltol=None):
+    """Times PCG solves (x0 = 0, rtol 1e-8) of several preconditioners on one
+    system: one warm-up solve each, then `rounds` rounds that solve once with
+    every configuration in turn (so clock and temperature drift affect them
+    alike); per configuration the median.  configs: [(label, P)]."""
+    times = {label: [] for label, _ in configs}
+    last = {}
+    for label, P in configs:
+        last[label] = hs.pcg_solve(M, b, Pl=P, reltol=1e-8, true_reltol=true_reltol)
+    for _ in range(rounds):
+        for label, P in configs:
+            sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8, true_reltol=true_reltol)
+            times[label].append(sol.seconds)
+            last[label] = sol
+    rows = []
+    for label, _ in configs:
+        sol, t = last[label], float(np.median(times[label]))
+        rows.append({"name": label, "iters": sol.iters, "status": sol.status,
+                     "true_relres": sol.true_relres, "ms_per_solve": round(t * 1e3, 3),
+                     "solves_per_s": round(1.0 / t, 3),
+                     "ms_per_iter": round(t * 1e3 / max(sol.iters, 1), 4),
+                     "ms_all_rounds": [round(v * 1e3, 3) for v in times[label]]})
+    return rows
 
 
 # The CPU baselines use one OpenMP thread per host core up to 16: the GPU
@@ -257,8 +268,9 @@ def serving_bench(args, log_fn=log):
 
 def mixed_precision_bench(hs, pb, builder, n3=192, log_fn=log):
     """3D 7-point variable-coefficient Poisson on n3^3 with lognormal face
-    coefficients (seed 2), b = U[-1,1] (seed 2): FP64 CG to rtol 1e-8 with an
-    L1GS partsize-64 symmetric smoother in FP64, FP32 and BF16 (off-diagonals),
+    coefficients (seed 2), b = U[-1,1] (seed 2): FP64 CG to a true relative
+    residual of 1e-8 with an L1GS partsize-64 symmetric smoother in FP64, FP32
+    and BF16 (off-diagonals), both triangle layouts, the solves interleaved;
     each reported with its final true residual and its iteration delta
     against the all-FP64 solve."""
     import torch
@@ -267,16 +279,16 @@ def mixed_precision_bench(hs, pb, builder, n3=192, log_fn=log):
     n = A.shape[0]
     M = hs.DeviceMatrix(A)
     b = torch.tensor(pb.uniform_rhs(n, 2), device="cuda")
-    rows = []
+    configs = []
     for cache, tag in ((hs.PackedBufferCache(), ""), (hs.PackedSymmetricCache(), " one-triangle")):
         for prec in ("fp64", "fp32", "bf16"):
-            log_fn(f"[bench] mixed precision: {prec}{tag} smoother")
+            log_fn(f"[bench] mixed precision: building the {prec}{tag} smoother")
             P = builder(A, 64, isSymA=True, sweep=hs.SymmetricSweep(), precision=prec,
                         cache_strategy=cache)
-            rows.append(pcg_bench(hs, M, b, P,
-                                  f"L1GS P=64 symmetric{tag}, {prec.upper()} smoother", reps=1,
-                                  true_reltol=1e-8))
-            del P
+            configs.append((f"L1GS P=64 symmetric{tag}, {prec.upper()} smoother", P))
+    log_fn("[bench] mixed precision: solves, interleaved")
+    rows = pcg_bench_interleaved(hs, M, b, configs, rounds=2, true_reltol=1e-8)
+    del configs
     base = rows[0]
     for k, r in enumerate(rows):
         same = rows[3 * (k // 3)]  # the FP64 solve with the same triangle storage
@@ -605,25 +617,25 @@ def main():
         log("[bench] PCG on 128^3 (rtol 1e-8)")
         M = hs.DeviceMatrix(A)
         bvec = torch.tensor(pb.uniform_rhs(n, 0), device="cuda")
-        pcg = [pcg_bench(hs, M, bvec, prec, "L1GS P=64 symmetric, FP64 smoother")]
+        # every preconditioner built first, then the solves interleaved round by round
+        configs = [("L1GS P=64 symmetric, FP64 smoother", prec)]
         for pname in ("fp32", "bf16"):
             pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname)
             pv.set_variant(int(best))
-            pcg.append(pcg_bench(hs, M, bvec, pv, f"L1GS P=64 symmetric, {pname.upper()} smoother"))
-            del pv
+            configs.append((f"L1GS P=64 symmetric, {pname.upper()} smoother", pv))
         for pname in ("fp64", "fp32", "bf16"):
             pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname,
                          cache_strategy=hs.PackedSymmetricCache())
-            pcg.append(pcg_bench(hs, M, bvec, pv,
-                                 f"L1GS P=64 symmetric one-triangle, {pname.upper()} smoother"))
-            del pv
+            configs.append((f"L1GS P=64 symmetric one-triangle, {pname.upper()} smoother", pv))
         c4 = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend()), A)
-        pcg.append(pcg_bench(hs, M, bvec, c4, "Chebyshev 4th kind degree 3, FP64"))
-        out["chebyshev_apply_128"] = cheb_apply_bench(hs, A, c4, x, y, args.steps, args.warmup,
-                                                      stream)
+        configs.append(("Chebyshev 4th kind degree 3, FP64", c4))
         c4f = hs._build_smoother(hs.ChebyshevFourth(degree=3, backend=hs.ROCBackend(),
                                                     precision="fp32"), A)
-        pcg.append(pcg_bench(hs, M, bvec, c4f, "Chebyshev 4th kind degree 3, FP32"))
+        configs.append(("Chebyshev 4th kind degree 3, FP32", c4f))
+        pcg = pcg_bench_interleaved(hs, M, bvec, configs, rounds=3)
+        out["chebyshev_apply_128"] = cheb_apply_bench(hs, A, c4, x, y, args.steps, args.warmup,
+                                                      stream)
+        del configs, c4f
         base = pcg[0]["solves_per_s"]
         base1 = next(r["solves_per_s"] for r in pcg if "one-triangle, FP64" in r["name"])
         for r in pcg:

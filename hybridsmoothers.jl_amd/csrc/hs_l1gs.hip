@@ -912,7 +912,8 @@ __device__ __forceinline__ double view_ring(double xr, double dr, double rr, int
         tk[k] = -1;
         if (k < w) {
             const int64_t p = e0 + 64 * k + lane;
-            const int64_t c = A.col16 ? s + lane + A.col16[p] : (int64_t)A.col[p];
+            const int64_t c =
+                (A.col16 || A.kbase) ? sell_col16(A, s + lane, p) : (int64_t)A.col[p];
             const int j = (int)(c - s);  // column within the partition
             int t = -1;
             if (MODE != MODE_BWD && j >= 0 && j < lane) t = j;

@@ -633,7 +633,10 @@ int hs_pcg_solve_true(hs_csr_t A, int prec_kind, void *prec, const double *b, do
         CGWork &w = work_for(*A);
         PrecRef M{prec_kind, prec};
         if (itmax < 0) itmax = 2 * n;
-        const int block = n < 200000 ? 16 : 4;
+        // iterations per graph replay between host checks of the done flag:
+        // launches past convergence exit at the flag (about 15 us an iteration
+        // at 128^3), a host check costs a copy and a synchronisation
+        const int block = 16;
         if (loc == HS_LOC_DEVICE) {
             pcg_device(*A, M, w, b, x, rtol, atol, itmax, block, true, user, stats, true_rtol);
         } else {

@@ -73,7 +73,9 @@ __device__ __forceinline__ bool cg_scalars_delta(CGDev &st, double delta) {
 // not loaded), so a wave keeps its loads in flight instead of one dependent
 // index -> value chain per entry; the products are still added one by one in
 // row order.
-template <bool C16, class TZ>
+// CF: 0 32-bit columns, 1 16-bit offsets from the row, 2 16-bit deltas from the
+// (slice, k) bases
+template <int CF, class TZ>
 __device__ __forceinline__ double cg_spmv_row_t(const SellView<double> &A,
                                                 const TZ *__restrict__ z,
                                                 const double *__restrict__ pold,
@@ -97,8 +99,12 @@ __device__ __forceinline__ double cg_spmv_row_t(const SellView<double> &A,
                 const int64_t p = p0 + 64 * (int64_t)(k0 + k);
                 // the matrix is streamed once per product: nontemporal, so it
                 // does not displace the gathered z and p from L2
-                if constexpr (C16) c[k] = (int32_t)(i + __builtin_nontemporal_load(A.col16 + p));
-                else c[k] = __builtin_nontemporal_load(A.col + p);
+                if constexpr (CF == 1)
+                    c[k] = (int32_t)(i + __builtin_nontemporal_load(A.col16 + p));
+                else if constexpr (CF == 2)
+                    c[k] = A.kbase[p >> 6] + (int32_t)__builtin_nontemporal_load(A.cdelta + p);
+                else
+                    c[k] = __builtin_nontemporal_load(A.col + p);
                 v[k] = __builtin_nontemporal_load(A.val + p);
             }
         }
@@ -129,8 +135,9 @@ __device__ __forceinline__ double cg_spmv_row(const SellView<double> &A,
                                               const double *__restrict__ pold,
                                               double *__restrict__ pnew, double *__restrict__ q,
                                               bool first, double beta, int64_t i) {
-    return A.col16 ? cg_spmv_row_t<true, TZ>(A, z, pold, pnew, q, first, beta, i)
-                   : cg_spmv_row_t<false, TZ>(A, z, pold, pnew, q, first, beta, i);
+    if (A.col16) return cg_spmv_row_t<1, TZ>(A, z, pold, pnew, q, first, beta, i);
+    if (A.kbase) return cg_spmv_row_t<2, TZ>(A, z, pold, pnew, q, first, beta, i);
+    return cg_spmv_row_t<0, TZ>(A, z, pold, pnew, q, first, beta, i);
 }
 
 // CG vector updates fused into a symmetric L1GS sweep: before the sweep each

@@ -129,6 +129,30 @@ __global__ __launch_bounds__(BLK) void k_sell_pack(int64_t n, int64_t nslices,
     if (!fits) atomicOr(wide, 1);
 }
 
+// The (slice, k) bases of the 16-bit delta format: the 64 entries of group
+// g = q >> 6 (slice s, position k) are one wave's; base = their smallest
+// column, delta = column - base; *wide2 is set when a group spans more than
+// 65,535 columns.
+__global__ __launch_bounds__(BLK) void k_sell_kbase(int64_t nent, const int32_t *__restrict__ col,
+                                                    int32_t *__restrict__ kbase,
+                                                    uint16_t *__restrict__ cdelta,
+                                                    int *__restrict__ wide2) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nent) return;  // nent is a multiple of 64: whole waves
+    const int32_t c = col[q];
+    int32_t lo = c, hi = c;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int32_t a = __shfl_xor(lo, off, 64), b = __shfl_xor(hi, off, 64);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if ((q & 63) == 0) kbase[q >> 6] = lo;
+    const int64_t d = (int64_t)c - lo;
+    cdelta[q] = (uint16_t)(d <= 65535 ? d : 0);
+    if ((q & 63) == 0 && (int64_t)hi - lo > 65535) atomicOr(wide2, 1);
+}
+
 }  // namespace
 
 void validate_sparse_device(int64_t n, int64_t nnz, const int64_t *ptr1, const int64_t *ind1,
@@ -225,7 +249,21 @@ void sell_build_device(CSR &A, int64_t n, int64_t nnz, const int64_t *rp1, const
     int h_wide = 0;
     HS_CHECK_HIP(hipMemcpyAsync(&h_wide, wide.get(), sizeof(int), hipMemcpyDeviceToHost, st));
     HS_CHECK_HIP(hipStreamSynchronize(st));  // the caller's inputs may go out of scope
-    if (h_wide) A.col16.release();  // some column is farther than 32,767 from its row
+    if (!h_wide) return;
+    A.col16.release();  // some column is farther than 32,767 from its row
+    // the (slice, k)-base format, if every group spans fewer than 65,536 columns
+    A.kbase.alloc(A.nent / 64);
+    A.cdelta.alloc(A.nent);
+    HS_CHECK_HIP(hipMemsetAsync(wide.get(), 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_sell_kbase, dim3(grid_for(A.nent, BLK)), dim3(BLK), 0, st, A.nent,
+                       A.col.get(), A.kbase.get(), A.cdelta.get(), wide.get());
+    check_launch();
+    HS_CHECK_HIP(hipMemcpyAsync(&h_wide, wide.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    HS_CHECK_HIP(hipStreamSynchronize(st));
+    if (h_wide) {  // 32-bit columns
+        A.kbase.release();
+        A.cdelta.release();
+    }
 }
 
 void csr_create_device(CSR &A, int64_t n, const int64_t *ptr, const int64_t *ind,

@@ -85,7 +85,7 @@ int hs_csr_info(hs_csr_t A, int64_t *out) {
 int hs_csr_column_bytes(hs_csr_t A, int64_t *bytes) {
     return guard([&] {
         HS_REQUIRE(A && bytes, HS_EINVAL, "NULL argument");
-        *bytes = A->col16.get() ? 2 : 4;
+        *bytes = A->column_bytes();
     });
 }
 

@@ -17,8 +17,12 @@ namespace hs {
 // comes last and adds zero), so results equal a row-by-row CSR product.
 // When every stored column is within 32,767 of its row (banded and stencil
 // matrices up to that bandwidth), the products read the column as a 16-bit
-// offset from the row (col16: 2 bytes per entry instead of 4); the 32-bit
-// columns are kept for the read-back and for matrices where it does not fit.
+// offset from the row (col16: 2 bytes per entry instead of 4).  Otherwise,
+// when in every slice the k-th entries of the 64 rows span fewer than 65,536
+// columns (stencils and banded matrices of wider bandwidth, e.g. the 192^3
+// grid's +-36,864 couplings), as a 16-bit delta from a 32-bit base per
+// (slice, k) (kbase, cdelta).  The 32-bit columns are kept for the read-back
+// and for matrices where neither fits.
 // The operator is built on the device (hs_setup.hip); the row lengths are
 // kept so that the CSR can be read back (csr_download).
 // ---------------------------------------------------------------------------
@@ -32,7 +36,18 @@ struct SellView {
     const T *val;
     int64_t n;
     const int16_t *col16 = nullptr;  // column - row, when every entry fits
+    // otherwise, when the k-th entries of every slice span fewer than 65,536
+    // columns: entry p's column is kbase[p >> 6] + cdelta[p] (slices start at
+    // multiples of 64 entries, so p >> 6 numbers the (slice, k) groups)
+    const int32_t *kbase = nullptr;
+    const uint16_t *cdelta = nullptr;
 };
+
+// column of stored entry p of row i in the 16-bit formats (kbase view or offset from the row)
+template <class T>
+__device__ __forceinline__ int64_t sell_col16(const SellView<T> &A, int64_t i, int64_t p) {
+    return A.col16 ? i + A.col16[p] : (int64_t)A.kbase[p >> 6] + A.cdelta[p];
+}
 
 struct CSR {
     uint64_t serial = next_serial();
@@ -43,6 +58,8 @@ struct CSR {
     DBuf<float> val32;
     DBuf<int32_t> rowlen;
     DBuf<int16_t> col16;
+    DBuf<int32_t> kbase;   // with cdelta: the 16-bit format of matrices col16 cannot hold
+    DBuf<uint16_t> cdelta;
     CGWork *cg = nullptr;
 
     CSR() = default;
@@ -52,17 +69,24 @@ struct CSR {
         if (cg) cgwork_free(cg);
     }
     void ensure_fp32(hipStream_t st);
-    SellView<double> view() const { return {sptr.get(), col.get(), val.get(), n, col16.get()}; }
+    SellView<double> view() const {
+        return {sptr.get(), col.get(), val.get(), n, col16.get(), kbase.get(), cdelta.get()};
+    }
     SellView<float> view32() const {
-        return {sptr.get(), col.get(), val32.get(), n, col16.get()};
+        return {sptr.get(), col.get(), val32.get(), n, col16.get(), kbase.get(), cdelta.get()};
     }
     template <class T> SellView<T> view_as() const {
         if constexpr (sizeof(T) == 8) return view();
         else return view32();
     }
+    // bytes per stored column as the products read it: 2 (offset from the row,
+    // or delta from the (slice, k) base) or 4
+    int column_bytes() const { return (col16.get() || kbase.get()) ? 2 : 4; }
     // algorithmic bytes of one product: values, column indices, slice offsets
+    // (and the (slice, k) bases)
     int64_t bytes(int vbytes) const {
-        return nent * (vbytes + (col16.get() ? 2 : 4)) + (nslices + 1) * 8;
+        return nent * (vbytes + column_bytes()) + (nslices + 1) * 8 +
+               (kbase.get() ? (nent / 64) * 4 : 0);
     }
 };
 
@@ -100,6 +124,9 @@ __device__ __forceinline__ void row_entries(const SellView<T> &A, int64_t i, F &
     const int64_t e = A.sptr[s + 1];
     if (A.col16) {
         for (int64_t p = A.sptr[s] + l; p < e; p += 64) f(A.val[p], i + A.col16[p]);
+    } else if (A.kbase) {
+        for (int64_t p = A.sptr[s] + l; p < e; p += 64)
+            f(A.val[p], (int64_t)A.kbase[p >> 6] + A.cdelta[p]);
     } else {
         for (int64_t p = A.sptr[s] + l; p < e; p += 64) f(A.val[p], (int64_t)A.col[p]);
     }

@@ -152,37 +152,46 @@ def test_operator_column_formats(gpu, far):
     np.testing.assert_array_equal(B.data, R.data)
 
 
-@pytest.mark.parametrize("far", [32767, 32768])
+@pytest.mark.parametrize("far", [32767, 32768, 70000])
 def test_column_format_boundary(gpu, far):
-    """A coupling exactly 32,767 rows away still fits the 16-bit column offsets
-    (2 bytes per stored column), one 32,768 away does not (4 bytes): on the
-    device-setup path (DeviceMatrix) and on the batched engine's upload, which
-    decide independently, and in the host bytes model the benchmark uses; the
-    products are the same either way."""
+    """Column formats at their boundaries.  A coupling exactly 32,767 rows away
+    still fits the 16-bit offsets from the row; one 32,768 away does not, and
+    the device operator then stores 16-bit deltas from a base per (slice, k)
+    -- the k-th entries of a 64-row slice span 32,768 columns -- while the
+    batched engine's upload falls back to 32-bit columns; one 70,000 away fits
+    neither 16-bit format.  The formats are decided on the device (the
+    operator) and on the host (the batched upload, and the bytes model the
+    benchmark uses), and the products are the same in every format."""
     import oracle as O
     from hybridsmoothers.batch import BatchSolver, column_bytes
     hs = gpu
-    n = 40000
+    n = 80000
     rng = np.random.default_rng(far)
     A = sp.diags([rng.uniform(-1, 1, n - 1), rng.uniform(-1, 1, n - far)], [1, far],
                  shape=(n, n), format="csc")
     A = (A + A.T + 8.0 * sp.identity(n)).tocsc()
     A.sort_indices()
-    want = 2 if far <= 32767 else 4
-    assert column_bytes(A.indptr + 1, A.indices + 1) == want
+    want_row = 2 if far <= 32767 else 4        # 16-bit offsets from the row
+    want_dev = 2 if far <= 65535 else 4        # ... or deltas from the (slice, k) base
+    assert column_bytes(A.indptr + 1, A.indices + 1) == want_row
     M = hs.DeviceMatrix(A)
-    assert M.column_bytes == want
+    assert M.column_bytes == want_dev
     x = rng.uniform(-1, 1, n)
     y = np.empty(n)
     M.mul(y, x)
     np.testing.assert_array_equal(y, O.spmv(O.csr1(A), x))
     b = rng.uniform(-1, 1, n)
+    P = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=64)(A, 64, isSymA=True)
+    ref = O.l1gs_build(A, 64, is_sym=True, sweep="symmetric", gpu=(256, 64))
+    _, so = O.pcg(A, b, ref, rtol=1e-8)
+    sol = hs.pcg_solve(M, b, Pl=P, reltol=1e-8)
+    assert sol.iters == so.iters
     for kw in (dict(partsize=32), dict(smoother="cheb", degree=2)):
-        with BatchSolver(precision="fp64", passes=4, arena_bytes=64 << 20, max_systems=2,
+        with BatchSolver(precision="fp64", passes=4, arena_bytes=128 << 20, max_systems=2,
                          **kw) as eng:
             sid = eng.submit(A, b)
             eng.step()
-            assert int(eng.peek(sid, "column_bytes")[0]) == want
+            assert int(eng.peek(sid, "column_bytes")[0]) == want_row
             while eng.pending():
                 eng.step()
                 eng.collect()
