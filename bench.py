@@ -558,6 +558,14 @@ def main():
         add_variant("128^3 P=128 symmetric fp64", lambda: hs.ldiv_(y, pw, x, sync=False),
                     sweep_bytes(n, pw.packed_size))
         del pw
+        # MatrixViewCache at partsize 128 and 256 (R = 2 and 4 rows per lane)
+        for pv_size in (128, 256):
+            pv = builder(A, pv_size, isSymA=True, sweep=hs.SymmetricSweep(),
+                         cache_strategy=hs.MatrixViewCache())
+            add_variant(f"128^3 P={pv_size} symmetric fp64 MatrixViewCache",
+                        lambda: hs.ldiv_(y, pv, x, sync=False),
+                        (n + 1) * 8 + int(A.nnz) * 16 + 3 * n * 8)
+            del pv
         # one stored triangle (U = L^T): same results, half the packed bytes
         for pname, tb, db in (("fp64", 8, 8), ("fp32", 4, 4), ("bf16", 2, 4)):
             pv = builder(A, P, isSymA=True, sweep=hs.SymmetricSweep(), precision=pname,

@@ -1021,6 +1021,272 @@ __global__ __launch_bounds__(64) void k_sweep_view_ring(const double *x, double 
     if (live) y[g] = yr;
 }
 
+// MatrixViewCache for partsize 65 to 256: one wave per partition in the
+// packed k_sweep_wide shape, lane l owning rows l, l + 64, ..., l + 64 (R - 1)
+// of the partition, and the right-looking recurrence of sweep_wide (the owner
+// of row j forms its quotient, it is broadcast, every row i past j adds
+// L(i, j) y_j, ascending j forward and descending backward).
+//
+// view_ring_wide: a row's entries inside the partition are contiguous in its
+// stored order (columns ascend), so each lane counts, from its row's columns
+// in the sliced-ELL copy, the entries before the partition and before its end,
+// and loads only that window (at most KS entries: value, and the step that
+// uses it -- forward step j for L(i, j), backward step 64 R - 1 - j for
+// U(i, j)).  The steps run in 64-step blocks, one per row slot, so the owner's
+// slot is known at compile time and rows already finished in a block take no
+// work; every partition runs all 64 R steps (rows past the end hold x = 0 and
+// no entries, and come after the live rows going forward and before them,
+// adding exact zeros, going back).  Within a block, groups of G steps read
+// their coefficients from a per-lane, per-row-slot LDS ring of G slots: a
+// group reads its slots, clears them and writes the entries of the next group.
+// A lane only reads and writes its own ring columns.  Slots without an entry
+// hold zero and finished rows take dead additions, as in the partsize-64 ring:
+// the results equal PackedBufferCache's bit for bit.  A wave with a row of
+// more than KS in-partition entries returns false (the caller takes the
+// queue form).
+namespace viewwide {
+template <int R>
+constexpr int G = R <= 2 ? 8 : 4;  // steps per group
+template <int R>
+constexpr int LG = R <= 2 ? 3 : 2;  // log2 G
+template <int R>
+constexpr int KS = R <= 2 ? 8 : 6;  // in-partition entries held per row
+constexpr int UPPER = 1024;         // step tag offset of the backward pass
+}  // namespace viewwide
+
+template <class T>
+__device__ __forceinline__ int64_t sell_col_at(const SellView<T> &A, int64_t row, int64_t p) {
+    return (A.col16 || A.kbase) ? sell_col16(A, row, p) : (int64_t)A.col[p];
+}
+
+template <int R, int MODE>
+__device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&dr)[R],
+                                               const double (&rr)[R], int lane, int Pa, int64_t s,
+                                               const SellView<double> &A, double *ring) {
+    using namespace viewwide;
+    constexpr int GS = G<R>, L2G = LG<R>, K = KS<R>, NB = 64 / GS;
+    double v[R][K];
+    int tk[R][K];
+    bool over = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        const int64_t g = s + i;
+        int64_t base = 0;
+        int klo = 0, khi = 0;
+        if (i < Pa) {
+            const int64_t sl = g >> 6;
+            base = A.sptr[sl] + (g & 63);
+            const int len = A.rowlen[g];
+#pragma unroll 1
+            for (int k = 0; k < len; ++k) {
+                const int64_t c = sell_col_at(A, g, base + 64 * (int64_t)k);
+                klo += c < s;
+                khi += c < s + Pa;
+            }
+        }
+        over = over || khi - klo > K;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int k = klo + q;
+            v[r][q] = 0.0;
+            tk[r][q] = -1;
+            if (k < khi) {
+                const int64_t p = base + 64 * (int64_t)k;
+                const int j = (int)(sell_col_at(A, g, p) - s);  // 0 <= j < Pa
+                if (MODE != MODE_BWD && j < i) tk[r][q] = j;
+                else if (MODE != MODE_FWD && j > i) tk[r][q] = UPPER + (64 * R - 1 - j);
+                v[r][q] = A.val[p];
+            }
+        }
+    }
+    if (__any(over)) return false;
+
+    double *const mine = ring + lane;  // slot u of row slot r: mine[64 (r GS + u)]
+    RowAcc<double> row[R];
+    double yfin[R];
+    auto pass = [&](auto Upper) {
+        constexpr bool UP = decltype(Upper)::value;
+        constexpr int TAG = UP ? UPPER : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            row[r] = RowAcc<double>(xr[r]);
+            yfin[r] = 0.0;
+#pragma unroll
+            for (int u = 0; u < GS; ++u) mine[64 * (r * GS + u)] = 0.0;
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+                if (tk[r][q] >= TAG && tk[r][q] < TAG + GS)
+                    mine[64 * (r * GS + (tk[r][q] & (GS - 1)))] = v[r][q];
+        }
+        static_for<0, R>([&](auto B) {
+            constexpr int b = decltype(B)::value;  // block of the pass
+            constexpr int js = UP ? R - 1 - b : b;  // row slot owning its steps
+            // row slots with terms left: past js going forward, up to js going back
+            constexpr int r0 = UP ? 0 : js, r1 = UP ? js + 1 : R;
+#pragma unroll 1
+            for (int gb = 0; gb < NB; ++gb) {
+                double cur[GS][R];
+#pragma unroll
+                for (int r = r0; r < r1; ++r)
+#pragma unroll
+                    for (int u = 0; u < GS; ++u) cur[u][r] = mine[64 * (r * GS + u)];
+                // the slots just read take the next group's entries
+                const int nxt = (TAG >> L2G) + b * NB + gb + 1;
+#pragma unroll
+                for (int r = r0; r < r1; ++r) {
+#pragma unroll
+                    for (int u = 0; u < GS; ++u) mine[64 * (r * GS + u)] = 0.0;
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
+                        if ((tk[r][q] >> L2G) == nxt && tk[r][q] >= 0)
+                            mine[64 * (r * GS + (tk[r][q] & (GS - 1)))] = v[r][q];
+                }
+#pragma unroll
+                for (int u = 0; u < GS; ++u) {
+                    const int jl = UP ? 63 - (gb * GS + u) : gb * GS + u;
+                    const double yj = bcast_keep<64>(row[js].q(dr[js], rr[js]), jl, yfin[js]);
+#pragma unroll
+                    for (int r = r0; r < r1; ++r) row[r].add(cur[u][r], yj);
+                }
+            }
+        });
+#pragma unroll
+        for (int r = 0; r < R; ++r) xr[r] = yfin[r];
+    };
+    if constexpr (MODE != MODE_BWD) pass(std::false_type{});
+    if constexpr (MODE == MODE_SYM) {
+        // y .= D_Dl1 .* y (:897); rows past the end restart from zero
+#pragma unroll
+        for (int r = 0; r < R; ++r) xr[r] = (lane + 64 * r < Pa) ? dr[r] * xr[r] : 0.0;
+    }
+    if constexpr (MODE != MODE_FWD) pass(std::true_type{});
+    return true;
+}
+
+// view_wave_wide: the general form (any row length, unsafe diagonals) on the
+// 1-based row arrays, view_wave's register queues with R rows per lane.
+template <int R, int MODE, bool FAST>
+__device__ __forceinline__ void view_wave_wide(double (&xr)[R], const double (&dr)[R],
+                                               const double (&rr)[R], int lane, int Pa, int64_t s,
+                                               const int64_t *__restrict__ ptr,
+                                               const int64_t *__restrict__ ind,
+                                               const double *__restrict__ val) {
+    int64_t plo[R], pmid[R], pmid1[R], phi[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        const int64_t g = s + i;
+        plo[r] = pmid[r] = pmid1[r] = phi[r] = 0;
+        if (i < Pa) {
+            const int64_t p0 = ptr[g] - 1, p1 = ptr[g + 1] - 1;
+            plo[r] = row_lower_bound(ind, p0, p1, s);
+            pmid[r] = row_lower_bound(ind, plo[r], p1, g);
+            pmid1[r] = (pmid[r] < p1 && ind[pmid[r]] - 1 == g) ? pmid[r] + 1 : pmid[r];
+            phi[r] = row_lower_bound(ind, pmid1[r], p1, s + Pa);
+        }
+    }
+    auto pass = [&](auto Upper) {
+        constexpr bool UP = decltype(Upper)::value;
+        RowAcc<double> row[R];
+        double yfin[R];
+        ViewCache vc[R];
+        int64_t p[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            row[r] = RowAcc<double>(xr[r]);
+            yfin[r] = 0.0;
+            p[r] = UP ? phi[r] - 1 : plo[r];
+            if (UP) view_fill(vc[r], p[r], -1, pmid1[r], phi[r], ind, val, s);
+            else view_fill(vc[r], p[r], 1, plo[r], pmid[r], ind, val, s);
+        }
+        for (int t = 0; t < Pa; ++t) {
+            const int j = UP ? Pa - 1 - t : t;
+            const int js = j >> 6, jl = j & 63;
+            double yj = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (r == js)
+                    yj = bcast_keep<64>(row[r].template q<FAST>(dr[r], rr[r]), jl, yfin[r]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (vc[r].c[0] == j) {
+                    row[r].add(vc[r].v[0], yj);
+                    p[r] += UP ? -1 : 1;
+#pragma unroll
+                    for (int k = 0; k + 1 < KV; ++k) {
+                        vc[r].c[k] = vc[r].c[k + 1];
+                        vc[r].v[k] = vc[r].v[k + 1];
+                    }
+                    vc[r].c[KV - 1] = -1;
+                    if (vc[r].c[0] < 0) {
+                        if (UP && p[r] >= pmid1[r])
+                            view_fill(vc[r], p[r], -1, pmid1[r], phi[r], ind, val, s);
+                        if (!UP && p[r] < pmid[r])
+                            view_fill(vc[r], p[r], 1, plo[r], pmid[r], ind, val, s);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) xr[r] = yfin[r];
+    };
+    if constexpr (MODE != MODE_BWD) pass(std::false_type{});
+    if constexpr (MODE == MODE_SYM) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) xr[r] = dr[r] * xr[r];  // y .= D_Dl1 .* y (:897)
+    }
+    if constexpr (MODE != MODE_FWD) pass(std::true_type{});
+}
+
+// RING: waves whose diagonals are safe take the ring; the others, and waves
+// with a row of too many in-partition entries, the queue form (true
+// divisions where a diagonal is unsafe; both round correctly, so the results
+// are the same).
+template <int R, int MODE, bool RING>
+__global__ __launch_bounds__(64) void k_sweep_view_wide(const double *x, double *y,
+                                                        SellView<double> A,
+                                                        const int64_t *__restrict__ ptr,
+                                                        const int64_t *__restrict__ ind,
+                                                        const double *__restrict__ val,
+                                                        const double *__restrict__ D, int64_t n,
+                                                        int64_t P, int64_t nparts,
+                                                        const int *__restrict__ done) {
+    if (done && *done) return;
+    __shared__ double ring[RING ? R * viewwide::G<R> * 64 : 1];
+    const int64_t k = blockIdx.x;
+    if (k >= nparts) return;
+    const int lane = threadIdx.x;
+    const int64_t s = k * P;
+    const int Pa = (int)(n - s < P ? n - s : P);
+    double xr[R], dr[R], rr[R];
+    bool unsafe = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        const bool live = i < Pa;
+        xr[r] = live ? x[s + i] : 0.0;
+        dr[r] = live ? D[s + i] : 1.0;
+        rr[r] = 1.0 / dr[r];
+        unsafe = unsafe || !safe_divisor(dr[r]);
+    }
+    bool done_ring = false;
+    if constexpr (RING) {
+        if (!__any(unsafe)) done_ring = view_ring_wide<R, MODE>(xr, dr, rr, lane, Pa, s, A, ring);
+    }
+    if (!done_ring) {
+        if (__any(unsafe))
+            view_wave_wide<R, MODE, false>(xr, dr, rr, lane, Pa, s, ptr, ind, val);
+        else
+            view_wave_wide<R, MODE, true>(xr, dr, rr, lane, Pa, s, ptr, ind, val);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        if (i < Pa) y[s + i] = xr[r];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
@@ -1041,7 +1307,8 @@ struct L1GSPlan {
     DBuf<double> vval;
     DBuf<int64_t> wptr, wind;
     DBuf<double> wval;
-    // MatrixViewCache at partsize 64: the rows as sliced ELL (k_sweep_view_ring)
+    // MatrixViewCache at partsize 64 to 256: the rows as sliced ELL
+    // (k_sweep_view_ring, k_sweep_view_wide)
     // and the widest slice's entry count
     CSR vsell;
     int vsell_w = 0;
@@ -1308,6 +1575,33 @@ static void launch_view_ring(const L1GSPlan &h, const double *x, double *y, cons
     else launch_view_ring_w<32>(h, x, y, rp, ri, rv, st, done);
 }
 
+template <int R, bool RING>
+static void launch_view_wide_r(const L1GSPlan &h, const double *x, double *y, const int64_t *rp,
+                               const int64_t *ri, const double *rv, hipStream_t st,
+                               const int *done) {
+    const dim3 grid(grid_for(h.nparts, 1)), block(64);
+    const SellView<double> A = h.vsell.view();
+    if (h.sweep == HS_SWEEP_FORWARD)
+        hipLaunchKernelGGL((k_sweep_view_wide<R, MODE_FWD, RING>), grid, block, 0, st, x, y, A,
+                           rp, ri, rv, h.D.get(), h.n, h.P, h.nparts, done);
+    else if (h.sweep == HS_SWEEP_BACKWARD)
+        hipLaunchKernelGGL((k_sweep_view_wide<R, MODE_BWD, RING>), grid, block, 0, st, x, y, A,
+                           rp, ri, rv, h.D.get(), h.n, h.P, h.nparts, done);
+    else
+        hipLaunchKernelGGL((k_sweep_view_wide<R, MODE_SYM, RING>), grid, block, 0, st, x, y, A,
+                           rp, ri, rv, h.D.get(), h.n, h.P, h.nparts, done);
+}
+
+// partsize 65..256: the ring kernel over the sliced-ELL rows (built at
+// partsizes 64..256), else the queue form throughout
+template <int R>
+static void launch_view_wide(const L1GSPlan &h, const double *x, double *y, const int64_t *rp,
+                             const int64_t *ri, const double *rv, hipStream_t st,
+                             const int *done) {
+    if (h.vsell.n == h.n) launch_view_wide_r<R, true>(h, x, y, rp, ri, rv, st, done);
+    else launch_view_wide_r<R, false>(h, x, y, rp, ri, rv, st, done);
+}
+
 // Device-side application y = M^-1 x (x, y device pointers; may alias).
 void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_t st,
                        const int *done) {
@@ -1317,6 +1611,13 @@ void l1gs_apply_device(const L1GSPlan &h, const double *x, double *y, hipStream_
         const int64_t *ri = h.rowwise ? h.vind.get() : h.wind.get();
         const double *rv = h.rowwise ? h.vval.get() : h.wval.get();
         launch_view_ring(h, x, y, rp, ri, rv, st, done);
+    } else if (h.cache == HS_CACHE_VIEW && h.P > 64 && h.P <= 256) {
+        const int64_t *rp = h.rowwise ? h.vptr.get() : h.wptr.get();
+        const int64_t *ri = h.rowwise ? h.vind.get() : h.wind.get();
+        const double *rv = h.rowwise ? h.vval.get() : h.wval.get();
+        if (h.P <= 128) launch_view_wide<2>(h, x, y, rp, ri, rv, st, done);
+        else if (h.P <= 192) launch_view_wide<3>(h, x, y, rp, ri, rv, st, done);
+        else launch_view_wide<4>(h, x, y, rp, ri, rv, st, done);
     } else if (h.cache == HS_CACHE_VIEW && h.P <= 64) {
         // rows: the matrix itself when it is read row-wise, else its transpose
         const int64_t *rp = h.rowwise ? h.vptr.get() : h.wptr.get();
@@ -1384,11 +1685,11 @@ int64_t l1gs_rows(const L1GSPlan &h) { return h.n; }
 int l1gs_precision(const L1GSPlan &h) { return h.prec; }
 uint64_t l1gs_serial(const L1GSPlan &h) { return h.serial; }
 
-// The sliced-ELL copy of the rows the partsize-64 MatrixViewCache kernel reads
-// (the matrix itself when read row-wise, else its transpose), and its widest
-// slice.
+// The sliced-ELL copy of the rows the MatrixViewCache ring kernels read at
+// partsize 64 to 256 (the matrix itself when read row-wise, else its
+// transpose), and its widest slice.
 static void build_view_sell(L1GSPlan &h, int64_t nnz, hipStream_t st) {
-    if (h.P != 64 || h.n == 0) return;
+    if (h.P < 64 || h.P > 256 || h.n == 0) return;
     const int64_t *rp = h.rowwise ? h.vptr.get() : h.wptr.get();
     const int64_t *ri = h.rowwise ? h.vind.get() : h.wind.get();
     const double *rv = h.rowwise ? h.vval.get() : h.wval.get();
@@ -1873,7 +2174,7 @@ int hs_l1gs_load(const char *path, void *stream, hs_l1gs_t *out) {
                 csc_transpose_device(n, nnz, h->vptr.get(), h->vind.get(), h->vval.get(), h->wptr,
                                      h->wind, h->wval, st);
             }
-            // the partsize-64 kernel's sliced-ELL rows are rebuilt, not stored
+            // the ring kernels' sliced-ELL rows are rebuilt, not stored
             build_view_sell(*h, nnz, st);
         }
         HS_CHECK_HIP(hipStreamSynchronize(st));  // the host buffers go out of scope

@@ -41,6 +41,7 @@ struct SellView {
     // multiples of 64 entries, so p >> 6 numbers the (slice, k) groups)
     const int32_t *kbase = nullptr;
     const uint16_t *cdelta = nullptr;
+    const int32_t *rowlen = nullptr;  // stored entries of each row (the rest is padding)
 };
 
 // column of stored entry p of row i in the 16-bit formats (kbase view or offset from the row)
@@ -70,10 +71,12 @@ struct CSR {
     }
     void ensure_fp32(hipStream_t st);
     SellView<double> view() const {
-        return {sptr.get(), col.get(), val.get(), n, col16.get(), kbase.get(), cdelta.get()};
+        return {sptr.get(), col.get(), val.get(), n, col16.get(), kbase.get(), cdelta.get(),
+                rowlen.get()};
     }
     SellView<float> view32() const {
-        return {sptr.get(), col.get(), val32.get(), n, col16.get(), kbase.get(), cdelta.get()};
+        return {sptr.get(), col.get(), val32.get(), n, col16.get(), kbase.get(), cdelta.get(),
+                rowlen.get()};
     }
     template <class T> SellView<T> view_as() const {
         if constexpr (sizeof(T) == 8) return view();

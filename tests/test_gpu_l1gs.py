@@ -288,6 +288,58 @@ def test_view_wave_equals_packed(gpu, kind, partsize, density):
         np.testing.assert_array_equal(hs.backslash(Pv, x), hs.backslash(Pp, x))
 
 
+@pytest.mark.parametrize("kind", ["sym_csc", "csr", "nonsym_csc"])
+@pytest.mark.parametrize("partsize,density", [(65, 0.004), (100, 0.006), (128, 0.004),
+                                              (128, 0.009), (128, 0.03), (160, 0.004),
+                                              (192, 0.006), (200, 0.004), (256, 0.004),
+                                              (256, 0.012)])
+def test_view_wide_equals_packed(gpu, kind, partsize, density):
+    """MatrixViewCache at partsize 65..256 (one wave per partition, R rows per
+    lane): rows of at most 8 stored entries (16 at R = 2) take the LDS step
+    ring fed from the sliced-ELL rows, wider rows the register-queue form; both
+    give the packed cache's results bit for bit.  Partitions do not start at
+    64-row slices for P = 65, 100, 160, 200; n = 599 leaves a ragged last
+    partition; densities 0.009 and 0.012 put 9..16 entries in some rows, 0.03
+    more than 16."""
+    hs = gpu
+    n = 64 * 9 + 23
+    A = _random_nonsym(n, density, partsize) if kind == "nonsym_csc" else \
+        _random_sym(n, density, partsize)
+    M = A.tocsr() if kind == "csr" else A
+    is_sym = kind != "nonsym_csc"
+    x = np.random.default_rng(partsize).uniform(-1, 1, n)
+    b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=64, blocks=5)
+    print(f"[view wide] {kind} P={partsize} density={density}", flush=True)
+    for sw in SWEEPS:
+        Pp = b(M, partsize, isSymA=is_sym, sweep=_hs_sweep(hs, sw))
+        Pv = b(M, partsize, isSymA=is_sym, sweep=_hs_sweep(hs, sw),
+               cache_strategy=hs.MatrixViewCache())
+        np.testing.assert_array_equal(hs.backslash(Pv, x), hs.backslash(Pp, x), err_msg=sw)
+
+
+@pytest.mark.parametrize("partsize", [96, 128, 192, 256])
+def test_view_wide_stencil_equals_packed(gpu, partsize):
+    """A 7-point lognormal grid (28 x 25 x 23: partitions cross grid lines and
+    planes at every partsize) through the wide MatrixViewCache ring, on device
+    vectors, against the packed cache bit for bit, in all three sweeps."""
+    import torch
+    from hybridsmoothers import problems as pb
+    hs = gpu
+    A = pb.csr_to_scipy(*pb.lognormal_grid(28, 25, 23, seed=4)).tocsc()
+    n = A.shape[0]
+    xd = torch.tensor(np.random.default_rng(3).uniform(-1, 1, n), device="cuda")
+    yp, yv = torch.empty_like(xd), torch.empty_like(xd)
+    b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=64)
+    print(f"[view wide stencil] n={n} P={partsize}", flush=True)
+    for sw in SWEEPS:
+        Pp = b(A, partsize, isSymA=True, sweep=_hs_sweep(hs, sw))
+        Pv = b(A, partsize, isSymA=True, sweep=_hs_sweep(hs, sw),
+               cache_strategy=hs.MatrixViewCache())
+        hs.ldiv_(yp, Pp, xd)
+        hs.ldiv_(yv, Pv, xd)
+        assert torch.equal(yp, yv), sw
+
+
 def test_c3_convdiff2048_parity(gpu):
     """2048^2 upwind convection-diffusion (4.19 M rows, non-symmetric CSC),
     DiagonalIndices path, P=32 forward and backward -- full size: D_Dl1 and
