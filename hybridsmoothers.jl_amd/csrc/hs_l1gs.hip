@@ -1028,9 +1028,9 @@ __global__ __launch_bounds__(64) void k_sweep_view_ring(const double *x, double 
 // L(i, j) y_j, ascending j forward and descending backward).
 //
 // view_ring_wide: a row's entries inside the partition are contiguous in its
-// stored order (columns ascend), so each lane counts, from its row's columns
-// in the sliced-ELL copy, the entries before the partition and before its end,
-// and loads only that window (at most KS entries: value, and the step that
+// stored order (columns ascend), so each lane counts, from its row's 32-bit
+// columns in the sliced-ELL copy, the entries before the partition and before
+// its end, and loads only that window (at most KS entries: value, and the step that
 // uses it -- forward step j for L(i, j), backward step 64 R - 1 - j for
 // U(i, j)).  The steps run in 64-step blocks, one per row slot, so the owner's
 // slot is known at compile time and rows already finished in a block take no
@@ -1038,71 +1038,94 @@ __global__ __launch_bounds__(64) void k_sweep_view_ring(const double *x, double 
 // no entries, and come after the live rows going forward and before them,
 // adding exact zeros, going back).  Within a block, groups of G steps read
 // their coefficients from a per-lane, per-row-slot LDS ring of G slots: a
-// group reads its slots, clears them and writes the entries of the next group.
+// group reads its slots, clears them and writes the entries of the next group
+// (every entry is written, to its slot or to a per-lane trash slot, so the
+// fill has no branches; the scan stops at the widest window of the wave).
 // A lane only reads and writes its own ring columns.  Slots without an entry
 // hold zero and finished rows take dead additions, as in the partsize-64 ring:
 // the results equal PackedBufferCache's bit for bit.  A wave with a row of
 // more than KS in-partition entries returns false (the caller takes the
 // queue form).
 namespace viewwide {
-template <int R>
-constexpr int G = R <= 2 ? 8 : 4;  // steps per group
-template <int R>
-constexpr int LG = R <= 2 ? 3 : 2;  // log2 G
+constexpr int G = 8;                // steps per group
+constexpr int LG = 3;               // log2 G
 template <int R>
 constexpr int KS = R <= 2 ? 8 : 6;  // in-partition entries held per row
 constexpr int UPPER = 1024;         // step tag offset of the backward pass
+template <int R>
+constexpr int ring_doubles() { return (R * G + 1) * 64; }  // + one trash slot per lane
 }  // namespace viewwide
-
-template <class T>
-__device__ __forceinline__ int64_t sell_col_at(const SellView<T> &A, int64_t row, int64_t p) {
-    return (A.col16 || A.kbase) ? sell_col16(A, row, p) : (int64_t)A.col[p];
-}
 
 template <int R, int MODE>
 __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&dr)[R],
                                                const double (&rr)[R], int lane, int Pa, int64_t s,
                                                const SellView<double> &A, double *ring) {
     using namespace viewwide;
-    constexpr int GS = G<R>, L2G = LG<R>, K = KS<R>, NB = 64 / GS;
+    constexpr int K = KS<R>, NB = 64 / G;
     double v[R][K];
     int tk[R][K];
-    bool over = false;
+    int cnt = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int i = lane + 64 * r;
         const int64_t g = s + i;
         int64_t base = 0;
-        int klo = 0, khi = 0;
+        int klo = 0, khi = 0, len = 0;
         if (i < Pa) {
             const int64_t sl = g >> 6;
             base = A.sptr[sl] + (g & 63);
-            const int len = A.rowlen[g];
+            len = A.rowlen[g];
+        }
+        // count the row's entries before the partition and before its end,
+        // eight (clamped, unconditional) column loads at a time
 #pragma unroll 1
-            for (int k = 0; k < len; ++k) {
-                const int64_t c = sell_col_at(A, g, base + 64 * (int64_t)k);
-                klo += c < s;
-                khi += c < s + Pa;
+        for (int k0 = 0; k0 < len; k0 += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + u < len ? k0 + u : len - 1;
+                const int64_t c = A.col[base + 64 * (int64_t)k];
+                klo += (k0 + u < len) & (c < s);
+                khi += (k0 + u < len) & (c < s + Pa);
             }
         }
-        over = over || khi - klo > K;
+        cnt = max(cnt, khi - klo);
+        // the window [klo, khi): entry klo + q, clamped loads
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const int k = klo + q;
-            v[r][q] = 0.0;
-            tk[r][q] = -1;
-            if (k < khi) {
-                const int64_t p = base + 64 * (int64_t)k;
-                const int j = (int)(sell_col_at(A, g, p) - s);  // 0 <= j < Pa
-                if (MODE != MODE_BWD && j < i) tk[r][q] = j;
-                else if (MODE != MODE_FWD && j > i) tk[r][q] = UPPER + (64 * R - 1 - j);
-                v[r][q] = A.val[p];
-            }
+            const bool in = klo + q < khi;
+            const int k = in ? klo + q : (len > 0 ? len - 1 : 0);
+            const int64_t p = len > 0 ? base + 64 * (int64_t)k : 0;
+            const int j = (int)((int64_t)A.col[p] - s);  // 0 <= j < Pa inside the window
+            const double val = A.val[p];
+            int t = -1;
+            if (in && MODE != MODE_BWD && j < i) t = j;
+            else if (in && MODE != MODE_FWD && j > i) t = UPPER + (64 * R - 1 - j);
+            tk[r][q] = t;
+            v[r][q] = t >= 0 ? val : 0.0;
         }
     }
-    if (__any(over)) return false;
+    if (__any(cnt > K)) return false;
+    // the widest window in the wave bounds the ring-fill scans (wave-uniform)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off));
+    const int kw = __builtin_amdgcn_readfirstlane(cnt);
 
-    double *const mine = ring + lane;  // slot u of row slot r: mine[64 (r GS + u)]
+    double *const mine = ring + lane;  // slot u of row slot r: mine[64 (r G + u)]
+    double *const trash = ring + R * G * 64 + lane;
+    // write each entry of the given group into its slot, the others into the trash slot
+    auto fill = [&](int grp, int rlo, int rhi) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (q >= kw) break;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (r < rlo || r >= rhi) continue;
+                const int t = tk[r][q];
+                double *const dst = (t >> LG) == grp ? mine + 64 * (r * G + (t & (G - 1))) : trash;
+                *dst = v[r][q];
+            }
+        }
+    };
     RowAcc<double> row[R];
     double yfin[R];
     auto pass = [&](auto Upper) {
@@ -1113,12 +1136,9 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
             row[r] = RowAcc<double>(xr[r]);
             yfin[r] = 0.0;
 #pragma unroll
-            for (int u = 0; u < GS; ++u) mine[64 * (r * GS + u)] = 0.0;
-#pragma unroll
-            for (int q = 0; q < K; ++q)
-                if (tk[r][q] >= TAG && tk[r][q] < TAG + GS)
-                    mine[64 * (r * GS + (tk[r][q] & (GS - 1)))] = v[r][q];
+            for (int u = 0; u < G; ++u) mine[64 * (r * G + u)] = 0.0;
         }
+        fill(TAG >> LG, 0, R);
         static_for<0, R>([&](auto B) {
             constexpr int b = decltype(B)::value;  // block of the pass
             constexpr int js = UP ? R - 1 - b : b;  // row slot owning its steps
@@ -1126,25 +1146,20 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
             constexpr int r0 = UP ? 0 : js, r1 = UP ? js + 1 : R;
 #pragma unroll 1
             for (int gb = 0; gb < NB; ++gb) {
-                double cur[GS][R];
+                double cur[G][R];
 #pragma unroll
                 for (int r = r0; r < r1; ++r)
 #pragma unroll
-                    for (int u = 0; u < GS; ++u) cur[u][r] = mine[64 * (r * GS + u)];
+                    for (int u = 0; u < G; ++u) cur[u][r] = mine[64 * (r * G + u)];
                 // the slots just read take the next group's entries
-                const int nxt = (TAG >> L2G) + b * NB + gb + 1;
 #pragma unroll
-                for (int r = r0; r < r1; ++r) {
+                for (int r = r0; r < r1; ++r)
 #pragma unroll
-                    for (int u = 0; u < GS; ++u) mine[64 * (r * GS + u)] = 0.0;
+                    for (int u = 0; u < G; ++u) mine[64 * (r * G + u)] = 0.0;
+                fill((TAG >> LG) + b * NB + gb + 1, r0, r1);
 #pragma unroll
-                    for (int q = 0; q < K; ++q)
-                        if ((tk[r][q] >> L2G) == nxt && tk[r][q] >= 0)
-                            mine[64 * (r * GS + (tk[r][q] & (GS - 1)))] = v[r][q];
-                }
-#pragma unroll
-                for (int u = 0; u < GS; ++u) {
-                    const int jl = UP ? 63 - (gb * GS + u) : gb * GS + u;
+                for (int u = 0; u < G; ++u) {
+                    const int jl = UP ? 63 - (gb * G + u) : gb * G + u;
                     const double yj = bcast_keep<64>(row[js].q(dr[js], rr[js]), jl, yfin[js]);
 #pragma unroll
                     for (int r = r0; r < r1; ++r) row[r].add(cur[u][r], yj);
@@ -1253,7 +1268,7 @@ __global__ __launch_bounds__(64) void k_sweep_view_wide(const double *x, double 
                                                         int64_t P, int64_t nparts,
                                                         const int *__restrict__ done) {
     if (done && *done) return;
-    __shared__ double ring[RING ? R * viewwide::G<R> * 64 : 1];
+    __shared__ double ring[RING ? viewwide::ring_doubles<R>() : 1];
     const int64_t k = blockIdx.x;
     if (k >= nparts) return;
     const int lane = threadIdx.x;
@@ -1598,7 +1613,8 @@ template <int R>
 static void launch_view_wide(const L1GSPlan &h, const double *x, double *y, const int64_t *rp,
                              const int64_t *ri, const double *rv, hipStream_t st,
                              const int *done) {
-    if (h.vsell.n == h.n) launch_view_wide_r<R, true>(h, x, y, rp, ri, rv, st, done);
+    if (h.vsell.n == h.n && h.vsell.nent > 0)
+        launch_view_wide_r<R, true>(h, x, y, rp, ri, rv, st, done);
     else launch_view_wide_r<R, false>(h, x, y, rp, ri, rv, st, done);
 }
 
