@@ -442,6 +442,8 @@ def main():
     ap.add_argument("--batch-steady", choices=("2d", "3d"), default=None,
                     help="only the batched engine's steady state (for kernel profiles)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 192^3 mixed-precision PCG")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="only the 192^3 mixed-precision PCG (for kernel profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only time a plain device copy and a sum over MIB MiB per step "
                          "(counter calibration, read-only stream rate)")
@@ -473,6 +475,11 @@ def main():
     hs.init(0)
     if args.batch_steady:
         print(json.dumps({"batch_steady": batch_steady(args.batch_steady)}), flush=True)
+        return
+    if args.mixed_only:
+        builder = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=256, blocks=1024)
+        print(json.dumps({"mixed_precision_192": mixed_precision_bench(hs, pb, builder)}),
+              flush=True)
         return
     if args.serve_only:
         res = serving_bench(args)
