@@ -871,18 +871,18 @@ __global__ __launch_bounds__(64) void k_sweep_view_wave(const double *x, double 
 // MatrixViewCache at partsize 64 from the sliced-ELL copy of the rows (the
 // device operator's layout, hs_sparse.hpp): partition k is slice k, so a
 // full-wave load of the slice's entry k gives lane i the k-th stored entry of
-// its own row (value, and its column as a 16-bit offset from the row when
-// every column fits).  The wave loads its rows' W <= WMAX entries once, maps
-// each in-partition entry to the recurrence step that uses it (forward step j
-// for L(i, j), backward step 126 - j for U(i, j) in the symmetric sweep), and
-// runs the packed kernels' step sequence with each step's coefficient taken
-// from a per-lane ring of NR step slots in LDS: every 8 steps a group of slots
-// that has been read is cleared and the entries of the group two ahead are
-// written into it.  A lane only ever reads and writes its own column of the
-// ring (slot * 64 + lane), so no cross-lane ordering is involved.  Slots
-// without an entry hold zero, so each step adds L(i, j) y_j or 0 * y_j exactly
-// as the packed triangles do: the results equal PackedBufferCache's bit for bit.
-// Matrix bytes read once per application: nnz (8 + 2) + 8 (slices + 1).
+// its own row (value, and its 32-bit column).  The wave loads its rows'
+// W <= WMAX entries once, maps each in-partition entry to the recurrence step
+// that uses it (forward step j for L(i, j), backward step 126 - j for U(i, j)
+// in the symmetric sweep), and runs the packed kernels' step sequence with
+// each step's coefficient taken from a per-lane ring of NR step slots in LDS:
+// every 8 steps, in the group of slots that has been read, the slots that
+// held an entry are zeroed and the entries of the group two ahead are written.
+// A lane only ever reads and writes its own column of the ring
+// (slot * 64 + lane), so no cross-lane ordering is involved.  Slots without an
+// entry hold zero, so each step adds L(i, j) y_j or 0 * y_j exactly as the
+// packed triangles do: the results equal PackedBufferCache's bit for bit.
+// Matrix bytes read once per application: nnz (8 + 4) + 8 (slices + 1).
 namespace viewring {
 constexpr int G = 8;    // steps per group
 constexpr int NR = 16;  // ring slots (two groups)
@@ -912,9 +912,7 @@ __device__ __forceinline__ double view_ring(double xr, double dr, double rr, int
         tk[k] = -1;
         if (k < w) {
             const int64_t p = e0 + 64 * k + lane;
-            const int64_t c =
-                (A.col16 || A.kbase) ? sell_col16(A, s + lane, p) : (int64_t)A.col[p];
-            const int j = (int)(c - s);  // column within the partition
+            const int j = (int)((int64_t)A.col[p] - s);  // column within the partition
             int t = -1;
             if (MODE != MODE_BWD && j >= 0 && j < lane) t = j;
             if (MODE == MODE_SYM && j > lane && j < 64) t = 126 - j;
@@ -924,12 +922,19 @@ __device__ __forceinline__ double view_ring(double xr, double dr, double rr, int
         }
     }
     double *const mine = ring + lane;  // slot u of this lane: mine[64 u]
+    // each entry's group and slot, once
+    int gk[WMAX], sk[WMAX];
+#pragma unroll
+    for (int k = 0; k < WMAX; ++k) {
+        gk[k] = tk[k] >= 0 ? tk[k] >> 3 : -1;
+        sk[k] = 64 * (tk[k] & (NR - 1));
+    }
     // groups 0 and 1 into the cleared ring
 #pragma unroll
     for (int u = 0; u < NR; ++u) mine[64 * u] = 0.0;
 #pragma unroll
     for (int k = 0; k < WMAX; ++k)
-        if (tk[k] >= 0 && tk[k] < NR) mine[64 * tk[k]] = v[k];
+        if (gk[k] == 0 || gk[k] == 1) mine[sk[k]] = v[k];
     double col[2][G];
 #pragma unroll
     for (int u = 0; u < G; ++u) col[0][u] = mine[64 * u];
@@ -940,13 +945,15 @@ __device__ __forceinline__ double view_ring(double xr, double dr, double rr, int
         constexpr int g = decltype(Gi)::value;
         constexpr int t0 = g * G;
         if constexpr ((g + 2) * G < NS) {
-            // group g's slots were read into registers: clear them and write
-            // the entries of group g + 2 (the same slots)
-#pragma unroll
-            for (int u = 0; u < G; ++u) mine[64 * ((t0 + u) % NR)] = 0.0;
+            // group g's slots were read into registers: zero the ones that
+            // held an entry, then write the entries of group g + 2 (the same
+            // slots); slots that held none are still zero
 #pragma unroll
             for (int k = 0; k < WMAX; ++k)
-                if ((tk[k] >> 3) == g + 2) mine[64 * (tk[k] & (NR - 1))] = v[k];
+                if (gk[k] == g) mine[sk[k]] = 0.0;
+#pragma unroll
+            for (int k = 0; k < WMAX; ++k)
+                if (gk[k] == g + 2) mine[sk[k]] = v[k];
         }
         if constexpr ((g + 1) * G < NS) {
 #pragma unroll
@@ -1038,9 +1045,10 @@ __global__ __launch_bounds__(64) void k_sweep_view_ring(const double *x, double 
 // no entries, and come after the live rows going forward and before them,
 // adding exact zeros, going back).  Within a block, groups of G steps read
 // their coefficients from a per-lane, per-row-slot LDS ring of G slots: a
-// group reads its slots, clears them and writes the entries of the next group
-// (every entry is written, to its slot or to a per-lane trash slot, so the
-// fill has no branches; the scan stops at the widest window of the wave).
+// group reads its slots, zeroes the ones that held an entry and writes the
+// entries of the next group (every entry is written, to its slot or to a
+// per-lane trash slot, so the fill has no branches; the scans stop at the
+// widest window of the wave).
 // A lane only reads and writes its own ring columns.  Slots without an entry
 // hold zero and finished rows take dead additions, as in the partsize-64 ring:
 // the results equal PackedBufferCache's bit for bit.  A wave with a row of
@@ -1112,8 +1120,9 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
 
     double *const mine = ring + lane;  // slot u of row slot r: mine[64 (r G + u)]
     double *const trash = ring + R * G * 64 + lane;
-    // write each entry of the given group into its slot, the others into the trash slot
-    auto fill = [&](int grp, int rlo, int rhi) {
+    // write each entry of the given group into its slot (its value, or zero to
+    // clear it), the others into the trash slot
+    auto fill = [&](int grp, int rlo, int rhi, bool zero) {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             if (q >= kw) break;
@@ -1122,7 +1131,7 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
                 if (r < rlo || r >= rhi) continue;
                 const int t = tk[r][q];
                 double *const dst = (t >> LG) == grp ? mine + 64 * (r * G + (t & (G - 1))) : trash;
-                *dst = v[r][q];
+                *dst = zero ? 0.0 : v[r][q];
             }
         }
     };
@@ -1138,7 +1147,7 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
 #pragma unroll
             for (int u = 0; u < G; ++u) mine[64 * (r * G + u)] = 0.0;
         }
-        fill(TAG >> LG, 0, R);
+        fill(TAG >> LG, 0, R, false);
         static_for<0, R>([&](auto B) {
             constexpr int b = decltype(B)::value;  // block of the pass
             constexpr int js = UP ? R - 1 - b : b;  // row slot owning its steps
@@ -1151,12 +1160,11 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
                 for (int r = r0; r < r1; ++r)
 #pragma unroll
                     for (int u = 0; u < G; ++u) cur[u][r] = mine[64 * (r * G + u)];
-                // the slots just read take the next group's entries
-#pragma unroll
-                for (int r = r0; r < r1; ++r)
-#pragma unroll
-                    for (int u = 0; u < G; ++u) mine[64 * (r * G + u)] = 0.0;
-                fill((TAG >> LG) + b * NB + gb + 1, r0, r1);
+                // the slots just read take the next group's entries: zero the
+                // ones that held an entry (the others are still zero), then fill
+                const int grp = (TAG >> LG) + b * NB + gb;
+                fill(grp, r0, r1, true);
+                fill(grp + 1, r0, r1, false);
 #pragma unroll
                 for (int u = 0; u < G; ++u) {
                     const int jl = UP ? 63 - (gb * G + u) : gb * G + u;

@@ -102,6 +102,12 @@ class ServeReport:
     latency_p99_ms: float
     latency_mean_ms: float
     latency_max_ms: float
+    # latency = wait (arrival to admission: generation, queueing behind
+    # resident systems) + service (admission to completion)
+    wait_p50_ms: float
+    wait_p99_ms: float
+    service_p50_ms: float
+    service_p99_ms: float
     rate: float
     iters_mean: float
     iters_max: int
@@ -155,6 +161,7 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
     own = producer is None
     prod = producer or Producer(specs)
     lat = np.full(N, np.nan)
+    t_in = np.full(N, np.nan)  # admission time, on the loop's clock
     iters = np.zeros(N, np.int64)
     relres = np.full(N, np.nan)
     status = [""] * N
@@ -194,6 +201,7 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
                 if engine.submit(A, b, rtol=rtol, tag=i) is None:
                     k += 1
                     continue
+                t_in[i] = time.perf_counter() - t0
                 del staged[i]
                 pending.pop(k)
                 rows += b.size
@@ -221,16 +229,23 @@ def serve(engine: BatchSolver, specs, arrivals: np.ndarray, rtol: float = 1e-8,
             prod.close()
     span = wall - float(arrivals[0])
     lat_ms = lat * 1e3
+    wait_ms = (t_in - arrivals[:N]) * 1e3
+    service_ms = lat_ms - wait_ms
     rate = float(N / arrivals[-1]) if arrivals[-1] > 0 else float("inf")
     total_bytes = int(np.sum(ibytes * (iters + 1)))  # + 1: the admission pass
     rep = ServeReport(count=N, wall_s=wall, solves_per_s=N / span if span > 0 else float("inf"),
                       latency_p50_ms=float(np.percentile(lat_ms, 50)),
                       latency_p99_ms=float(np.percentile(lat_ms, 99)),
                       latency_mean_ms=float(np.mean(lat_ms)), latency_max_ms=float(np.max(lat_ms)),
+                      wait_p50_ms=float(np.percentile(wait_ms, 50)),
+                      wait_p99_ms=float(np.percentile(wait_ms, 99)),
+                      service_p50_ms=float(np.percentile(service_ms, 50)),
+                      service_p99_ms=float(np.percentile(service_ms, 99)),
                       rate=rate, iters_mean=float(np.mean(iters)), iters_max=int(np.max(iters)),
                       max_true_relres=float(np.nanmax(relres)),
                       all_solved=all(s == "solved" for s in status), rows_total=int(rows),
                       steps=engine.info()["steps"] - steps0, algorithmic_bytes=total_bytes,
                       achieved_GBs=total_bytes / wall / 1e9)
-    records = {"latency_s": lat, "iters": iters, "true_relres": relres, "status": status, "x": xs}
+    records = {"latency_s": lat, "wait_s": wait_ms * 1e-3, "service_s": service_ms * 1e-3,
+               "iters": iters, "true_relres": relres, "status": status, "x": xs}
     return rep, records

@@ -94,10 +94,13 @@ def test_chebyshev_amg_smoother(gpu):
     np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize("sweep", ["forward", "backward", "symmetric"])
-@pytest.mark.parametrize("partsize,cache,iters", [(3, "packed", 1), (32, "packed", 2),
-                                                  (64, "packed", 3), (64, "view", 2),
-                                                  (100, "packed", 2), (16, "sym1", 2)])
+_ROUNDS = [(sw, p, c, it) for sw in ("forward", "backward", "symmetric")
+           for p, c, it in [(3, "packed", 1), (32, "packed", 2), (64, "packed", 3),
+                            (64, "view", 2), (100, "packed", 2), (128, "view", 3)]]
+_ROUNDS.append(("symmetric", 16, "sym1", 2))  # the one-triangle layout is symmetric-only
+
+
+@pytest.mark.parametrize("sweep,partsize,cache,iters", _ROUNDS)
 def test_fused_smoother_rounds(gpu, sweep, partsize, cache, iters):
     """Each round of hs_l1gs_smooth is one fused launch for packed layouts with
     partsize <= 64 (residual in the sweep prologue, x - z stored to the other
@@ -107,8 +110,6 @@ def test_fused_smoother_rounds(gpu, sweep, partsize, cache, iters):
     import torch
     from hybridsmoothers import amg, problems as pb
     hs = gpu
-    if cache == "sym1" and sweep != "symmetric":
-        pytest.skip("the one-triangle cache is a symmetric-sweep layout")
     A = pb.lognormal3d(14, seed=12)
     n = A.shape[0]
     sw = {"forward": hs.ForwardSweep(), "backward": hs.BackwardSweep(),

@@ -176,6 +176,11 @@ def test_serve_stream_alone_vs_mix(gpu):
         assert rep.count == N and rep.all_solved
         assert np.all(np.isfinite(rec["latency_s"])) and np.all(rec["latency_s"] >= 0)
         assert rep.latency_p50_ms <= rep.latency_p99_ms <= rep.latency_max_ms
+        # latency splits into admission wait and service, both non-negative
+        assert np.all(rec["wait_s"] >= 0) and np.all(rec["service_s"] > 0)
+        np.testing.assert_allclose(rec["wait_s"] + rec["service_s"], rec["latency_s"],
+                                   rtol=0, atol=1e-9)
+        assert rep.wait_p99_ms <= rep.latency_max_ms and rep.service_p99_ms <= rep.latency_max_ms
         assert rep.max_true_relres < 1e-6
         systems = autotune.sample_systems(specs)  # same right-hand sides as the stream
         for i in range(0, N, 3):
