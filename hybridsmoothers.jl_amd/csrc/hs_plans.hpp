@@ -69,10 +69,10 @@ __device__ __forceinline__ bool cg_scalars_delta(CGDev &st, double delta) {
 // formed on the fly for row i and its columns, q_i = (A p_k)_i in row order.
 // Stores p_k[i] and q[i]; returns p_k[i] * q_i (the p'q partial of the row).
 // The row's entries are taken in batches of 8: all column indices and values
-// of a batch are loaded before any gather (entries past the slice width are
-// not loaded), so a wave keeps its loads in flight instead of one dependent
-// index -> value chain per entry; the products are still added one by one in
-// row order.
+// of a batch are loaded before any gather (entries past the slice width load
+// the slice's last entry again and are not added), so a wave keeps its loads
+// in flight instead of one dependent index -> value chain per entry; the
+// products are still added one by one in row order.
 // CF: 0 32-bit columns, 1 16-bit offsets from the row, 2 16-bit deltas from the
 // (slice, k) bases
 template <int CF, class TZ>
@@ -91,22 +91,22 @@ __device__ __forceinline__ double cg_spmv_row_t(const SellView<double> &A,
         double v[B], zc[B], pc[B];
 #pragma unroll
         for (int k = 0; k < B; ++k) {
-            // past the slice width (the same in the whole wave) no load: the
-            // row's own column and a zero that is never added
-            c[k] = (int32_t)i;
-            v[k] = 0.0;
-            if (k0 + k < w) {
-                const int64_t p = p0 + 64 * (int64_t)(k0 + k);
-                // the matrix is streamed once per product: nontemporal, so it
-                // does not displace the gathered z and p from L2
-                if constexpr (CF == 1)
-                    c[k] = (int32_t)(i + __builtin_nontemporal_load(A.col16 + p));
-                else if constexpr (CF == 2)
-                    c[k] = A.kbase[p >> 6] + (int32_t)__builtin_nontemporal_load(A.cdelta + p);
-                else
-                    c[k] = __builtin_nontemporal_load(A.col + p);
-                v[k] = __builtin_nontemporal_load(A.val + p);
-            }
+            // every load is issued, with no branch around it: past the slice
+            // width (the same in the whole wave) the slice's last entry is
+            // loaded again (its lines are in cache) and never added.  Loads
+            // under a per-entry branch were each waited for inside the branch,
+            // so the batch's entries reached memory one after another.
+            const int ke = (k0 + k < w) ? k0 + k : w - 1;
+            const int64_t p = p0 + 64 * (int64_t)ke;
+            // the matrix is streamed once per product: nontemporal, so it
+            // does not displace the gathered z and p from L2
+            if constexpr (CF == 1)
+                c[k] = (int32_t)(i + __builtin_nontemporal_load(A.col16 + p));
+            else if constexpr (CF == 2)
+                c[k] = A.kbase[p >> 6] + (int32_t)__builtin_nontemporal_load(A.cdelta + p);
+            else
+                c[k] = __builtin_nontemporal_load(A.col + p);
+            v[k] = __builtin_nontemporal_load(A.val + p);
         }
 #pragma unroll
         for (int k = 0; k < B; ++k) {
