@@ -910,15 +910,24 @@ __device__ __forceinline__ double view_ring(double xr, double dr, double rr, int
     for (int k = 0; k < WMAX; ++k) {
         v[k] = 0.0;
         tk[k] = -1;
-        if (k < w) {
-            const int64_t p = e0 + 64 * k + lane;
+    }
+    if (w > 0) {  // wave-uniform
+        // every load is issued with no branch around it (loads under a
+        // per-entry branch were each waited for inside it): past the slice
+        // width the slice's last entry is loaded again and not used
+#pragma unroll
+        for (int k = 0; k < WMAX; ++k) {
+            const int64_t p = e0 + 64 * (k < w ? k : w - 1) + lane;
             const int j = (int)((int64_t)A.col[p] - s);  // column within the partition
+            const double val = A.val[p];
             int t = -1;
             if (MODE != MODE_BWD && j >= 0 && j < lane) t = j;
             if (MODE == MODE_SYM && j > lane && j < 64) t = 126 - j;
             if (MODE == MODE_BWD && j > lane && j < 64) t = 63 - j;
-            tk[k] = t;
-            v[k] = A.val[p];
+            if (k < w) {
+                tk[k] = t;
+                v[k] = val;
+            }
         }
     }
     double *const mine = ring + lane;  // slot u of this lane: mine[64 u]
