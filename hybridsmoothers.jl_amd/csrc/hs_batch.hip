@@ -448,8 +448,9 @@ __global__ __launch_bounds__(256) void k_bc_first(const Tile *__restrict__ tiles
             rdst[i] = rn;
             S.x[i] = init ? 0.0 : S.x[i] + alpha * p[i];
             const SellView<T> A = cheb_view<T>(S);
-            T ri = (T)rn;
-            row_entries(A, i, [&](T a, int64_t c) { ri = ri - a * z1_at(c); });
+            const T ri = row_fold(
+                A, i, (T)rn, [&](int64_t c) { return z1_at(c); },
+                [](T acc, T a, T z1) { return acc - a * z1; });
             const T zi = z1_at(i);
             double xi = 0.0 + (double)zi;
             const T zn = a2 * zi + b2 * l1[i] * ri;

@@ -117,37 +117,64 @@ void csr_download(const CSR &A, int64_t *rowptr1, int64_t *colval1, double *nzva
 // ---------------------------------------------------------------------------
 // row products, summed in row order
 // ---------------------------------------------------------------------------
-// Calls f(value, column) for the stored entries of row i in order (padding
-// last); the column format is the same for the whole matrix, so the branch
-// is uniform.
-template <class T, class F>
-__device__ __forceinline__ void row_entries(const SellView<T> &A, int64_t i, F &&f) {
+// Folds the stored entries of row i in order (padding last):
+// acc = f(acc, value, gather(column)) entry by entry.  The entries are taken
+// eight at a time: all column indices and values of a batch are loaded first,
+// with no branch around the loads (past the slice width, the same in the whole
+// wave, the slice's last entry is loaded again and not folded), then the
+// batch's gathers are issued together, then the terms are folded in row
+// order.  The loop form (load, gather, fold, one entry after the other) left a
+// wave one dependent load chain per entry.  The column format is the same
+// for the whole matrix, so the branch on it is uniform.
+template <int CF, class T, class R, class G, class F>
+__device__ __forceinline__ R row_fold_t(const SellView<T> &A, int64_t i, R acc, G &&gather,
+                                        F &&f) {
+    constexpr int B = 8;
     const int64_t s = i >> 6;
-    const int l = (int)(i & 63);
-    const int64_t e = A.sptr[s + 1];
-    if (A.col16) {
-        for (int64_t p = A.sptr[s] + l; p < e; p += 64) f(A.val[p], i + A.col16[p]);
-    } else if (A.kbase) {
-        for (int64_t p = A.sptr[s] + l; p < e; p += 64)
-            f(A.val[p], (int64_t)A.kbase[p >> 6] + A.cdelta[p]);
-    } else {
-        for (int64_t p = A.sptr[s] + l; p < e; p += 64) f(A.val[p], (int64_t)A.col[p]);
+    const int64_t s0 = A.sptr[s];
+    const int64_t p0 = s0 + (i & 63);
+    const int w = (int)((A.sptr[s + 1] - s0) >> 6);  // slice width, same in the wave
+    for (int k0 = 0; k0 < w; k0 += B) {
+        int64_t c[B];
+        T v[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int64_t p = p0 + 64 * (int64_t)((k0 + k < w) ? k0 + k : w - 1);
+            if constexpr (CF == 1) c[k] = i + A.col16[p];
+            else if constexpr (CF == 2) c[k] = (int64_t)A.kbase[p >> 6] + A.cdelta[p];
+            else c[k] = (int64_t)A.col[p];
+            v[k] = A.val[p];
+        }
+        decltype(gather(c[0])) g[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) g[k] = gather(c[k]);
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (k0 + k < w) acc = f(acc, v[k], g[k]);
     }
+    return acc;
+}
+template <class T, class R, class G, class F>
+__device__ __forceinline__ R row_fold(const SellView<T> &A, int64_t i, R acc, G &&gather,
+                                      F &&f) {
+    if (A.col16) return row_fold_t<1>(A, i, acc, gather, f);
+    if (A.kbase) return row_fold_t<2>(A, i, acc, gather, f);
+    return row_fold_t<0>(A, i, acc, gather, f);
 }
 
 template <class T, class TX>
 __device__ __forceinline__ T row_dot(const SellView<T> &A, const TX *__restrict__ x, int64_t i) {
-    T acc = (T)0;
-    row_entries(A, i, [&](T a, int64_t c) { acc = acc + a * (T)x[c]; });
-    return acc;
+    return row_fold(
+        A, i, (T)0, [&](int64_t c) { return (T)x[c]; },
+        [](T acc, T a, T xc) { return acc + a * xc; });
 }
 
 // r_i - a_1 x_1 - a_2 x_2 - ... term by term (5-argument mul! with alpha = -1, beta = 1)
 template <class T, class TX>
 __device__ __forceinline__ T row_sub(const SellView<T> &A, const TX *__restrict__ x, int64_t i,
                                      T r) {
-    row_entries(A, i, [&](T a, int64_t c) { r = r - a * (T)x[c]; });
-    return r;
+    return row_fold(
+        A, i, r, [&](int64_t c) { return (T)x[c]; }, [](T acc, T a, T xc) { return acc - a * xc; });
 }
 
 // sum_j |a_ij| in row order
