@@ -874,18 +874,28 @@ __global__ __launch_bounds__(64) void k_sweep_view_wave(const double *x, double 
 // its own row (value, and its 32-bit column).  The wave loads its rows'
 // W <= WMAX entries once, maps each in-partition entry to the recurrence step
 // that uses it (forward step j for L(i, j), backward step 126 - j for U(i, j)
-// in the symmetric sweep), and runs the packed kernels' step sequence with
-// each step's coefficient taken from a per-lane ring of NR step slots in LDS:
-// every 8 steps, in the group of slots that has been read, the slots that
-// held an entry are zeroed and the entries of the group two ahead are written.
+// in the symmetric sweep), keeps those that have a step in C <= WMAX slots
+// (C = 2, 4 or WMAX, the fewest that hold every lane's), and runs the packed
+// kernels' step sequence with each step's coefficient taken from a per-lane
+// ring of NR = 8 step slots in LDS (4 KB): every G = 4 steps, in the group of
+// slots that has been read, the slots that held an entry are zeroed and the
+// entries of the group two ahead are written.
 // A lane only ever reads and writes its own column of the ring
 // (slot * 64 + lane), so no cross-lane ordering is involved.  Slots without an
 // entry hold zero, so each step adds L(i, j) y_j or 0 * y_j exactly as the
 // packed triangles do: the results equal PackedBufferCache's bit for bit.
+// A lane's own quotient is not kept at its step: the row sum starts at +0 and
+// a rounded sum is -0 only when both addends are, so the sum is never -0, and
+// adding 0 * y_j = +-0 (y_j finite) leaves it unchanged bit for bit.  Each
+// lane's quotient formed once after the last step is therefore the one of its
+// own step.  A non-finite y_j makes 0 * y_j a NaN; it also leaves the output
+// of lane j (or of a lane that took the NaN) non-finite, and the caller then
+// redoes the wave by rows (view_wave), which adds stored entries only.
 // Matrix bytes read once per application: nnz (8 + 4) + 8 (slices + 1).
 namespace viewring {
-constexpr int G = 8;    // steps per group
-constexpr int NR = 16;  // ring slots (two groups)
+constexpr int LG = 2;
+constexpr int G = 1 << LG;  // steps per group
+constexpr int NR = 2 * G;   // ring slots (two groups)
 template <int MODE>
 constexpr int nstep() { return MODE == MODE_SYM ? 126 : 63; }
 // column of step t
@@ -895,14 +905,97 @@ __host__ __device__ constexpr int step_col(int t) {
 }
 }  // namespace viewring
 
-template <int MODE, int WMAX>
-__device__ __forceinline__ double view_ring(double xr, double dr, double rr, int lane, int64_t s,
-                                            const SellView<double> &A, int64_t e0, int w,
-                                            double *ring) {
+// The ring run over a lane's C entries (steps tk, -1: none; values v).
+template <int MODE, int C>
+__device__ __forceinline__ double view_ring_run(double xr, double dr, double rr, int lane,
+                                                const int (&tk)[C], const double (&v)[C],
+                                                double *ring) {
     using namespace viewring;
     constexpr int NS = nstep<MODE>();
     constexpr int NG = (NS + G - 1) / G;
     static_assert(NR == 2 * G, "the ring holds two groups");
+    double *const mine = ring + lane;  // slot u of this lane: mine[64 u]
+    // each entry's group and slot, once
+    int gk[C], sk[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        gk[k] = tk[k] >= 0 ? tk[k] >> LG : -1;
+        sk[k] = 64 * (tk[k] & (NR - 1));
+    }
+    // groups 0 and 1 into the cleared ring
+#pragma unroll
+    for (int u = 0; u < NR; ++u) mine[64 * u] = 0.0;
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+        if (gk[k] == 0 || gk[k] == 1) mine[sk[k]] = v[k];
+    double col[2][G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) col[0][u] = mine[64 * u];
+
+    RowAcc<double> row(xr);
+    static_for<0, NG>([&](auto Gi) {
+        constexpr int g = decltype(Gi)::value;
+        constexpr int t0 = g * G;
+        if constexpr ((g + 2) * G < NS) {
+            // group g's slots were read into registers: zero the ones that
+            // held an entry, then write the entries of group g + 2 (the same
+            // slots); slots that held none are still zero
+#pragma unroll
+            for (int k = 0; k < C; ++k)
+                if (gk[k] == g) mine[sk[k]] = 0.0;
+#pragma unroll
+            for (int k = 0; k < C; ++k)
+                if (gk[k] == g + 2) mine[sk[k]] = v[k];
+        }
+        if constexpr ((g + 1) * G < NS) {
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (t0 + G + u < NS) col[(g + 1) & 1][u] = mine[64 * ((t0 + G + u) % NR)];
+        }
+        static_for<t0, t0 + G>([&](auto T) {
+            constexpr int t = decltype(T)::value;
+            if constexpr (t < NS) {
+                if constexpr (MODE == MODE_SYM && t == 63) {
+                    // every row's forward quotient, y .= D_Dl1 .* y (:897),
+                    // then the backward pass
+                    row = RowAcc<double>(dr * row.q(dr, rr));
+                }
+                constexpr int j = step_col<MODE>(t);
+                const double yj = seg_bcast<64>(row.q(dr, rr), j);
+                row.add(col[g & 1][t - t0], yj);  // zero for lanes without a term
+            }
+        });
+    });
+    return row.q(dr, rr);
+}
+
+// The first C entries with a step, in stored order (at most C per lane).
+template <int C, int W>
+__device__ __forceinline__ void view_ring_compact(const int (&tk)[W], const double (&v)[W],
+                                                  int (&ct)[C], double (&cv)[C]) {
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+        ct[q] = -1;
+        cv[q] = 0.0;
+    }
+#pragma unroll
+    for (int k = W - 1; k >= 0; --k) {
+        if (tk[k] >= 0) {
+#pragma unroll
+            for (int q = C - 1; q > 0; --q) {
+                ct[q] = ct[q - 1];
+                cv[q] = cv[q - 1];
+            }
+            ct[0] = tk[k];
+            cv[0] = v[k];
+        }
+    }
+}
+
+template <int MODE, int WMAX>
+__device__ __forceinline__ double view_ring(double xr, double dr, double rr, int lane, int64_t s,
+                                            const SellView<double> &A, int64_t e0, int w,
+                                            double *ring) {
     // this lane's entries, their steps (-1: none) and values
     double v[WMAX];
     int tk[WMAX];
@@ -911,6 +1004,7 @@ __device__ __forceinline__ double view_ring(double xr, double dr, double rr, int
         v[k] = 0.0;
         tk[k] = -1;
     }
+    int cnt = 0;
     if (w > 0) {  // wave-uniform
         // every load is issued with no branch around it (loads under a
         // per-entry branch were each waited for inside it): past the slice
@@ -927,68 +1021,33 @@ __device__ __forceinline__ double view_ring(double xr, double dr, double rr, int
             if (k < w) {
                 tk[k] = t;
                 v[k] = val;
+                cnt += t >= 0;
             }
         }
     }
-    double *const mine = ring + lane;  // slot u of this lane: mine[64 u]
-    // each entry's group and slot, once
-    int gk[WMAX], sk[WMAX];
-#pragma unroll
-    for (int k = 0; k < WMAX; ++k) {
-        gk[k] = tk[k] >= 0 ? tk[k] >> 3 : -1;
-        sk[k] = 64 * (tk[k] & (NR - 1));
+    // the ring upkeep costs a compare per held entry and group: hold only the
+    // entries with a step (a row of the 3D Poisson stencil has two, its
+    // neighbours on the same grid line), in the fewest slots that fit every
+    // lane of the wave
+    if (!__any(cnt > 2)) {
+        int ct[2];
+        double cv[2];
+        view_ring_compact<2>(tk, v, ct, cv);
+        return view_ring_run<MODE, 2>(xr, dr, rr, lane, ct, cv, ring);
     }
-    // groups 0 and 1 into the cleared ring
-#pragma unroll
-    for (int u = 0; u < NR; ++u) mine[64 * u] = 0.0;
-#pragma unroll
-    for (int k = 0; k < WMAX; ++k)
-        if (gk[k] == 0 || gk[k] == 1) mine[sk[k]] = v[k];
-    double col[2][G];
-#pragma unroll
-    for (int u = 0; u < G; ++u) col[0][u] = mine[64 * u];
+    if constexpr (WMAX > 4) {
+        if (!__any(cnt > 4)) {
+            int ct[4];
+            double cv[4];
+            view_ring_compact<4>(tk, v, ct, cv);
+            return view_ring_run<MODE, 4>(xr, dr, rr, lane, ct, cv, ring);
+        }
+    }
+    return view_ring_run<MODE, WMAX>(xr, dr, rr, lane, tk, v, ring);
+}
 
-    RowAcc<double> row(xr);
-    double yfin = 0.0;
-    static_for<0, NG>([&](auto Gi) {
-        constexpr int g = decltype(Gi)::value;
-        constexpr int t0 = g * G;
-        if constexpr ((g + 2) * G < NS) {
-            // group g's slots were read into registers: zero the ones that
-            // held an entry, then write the entries of group g + 2 (the same
-            // slots); slots that held none are still zero
-#pragma unroll
-            for (int k = 0; k < WMAX; ++k)
-                if (gk[k] == g) mine[sk[k]] = 0.0;
-#pragma unroll
-            for (int k = 0; k < WMAX; ++k)
-                if (gk[k] == g + 2) mine[sk[k]] = v[k];
-        }
-        if constexpr ((g + 1) * G < NS) {
-#pragma unroll
-            for (int u = 0; u < G; ++u)
-                if (t0 + G + u < NS) col[(g + 1) & 1][u] = mine[64 * ((t0 + G + u) % NR)];
-        }
-        static_for<t0, t0 + G>([&](auto T) {
-            constexpr int t = decltype(T)::value;
-            if constexpr (t < NS) {
-                if constexpr (MODE == MODE_SYM && t == 63) {
-                    // last forward row, y .= D_Dl1 .* y (:897), then the backward pass
-                    const double q = row.q(dr, rr);
-                    const double yf = (lane == 63) ? q : yfin;
-                    row = RowAcc<double>(dr * yf);
-                    yfin = 0.0;
-                }
-                constexpr int j = step_col<MODE>(t);
-                const double q = row.q(dr, rr);
-                const double yj = bcast_keep<64>(q, j, yfin);
-                row.add(col[g & 1][t - t0], yj);  // zero, or dead, for lanes without a term
-            }
-        });
-    });
-    const double q = row.q(dr, rr);
-    if constexpr (MODE == MODE_FWD) return (lane == 63) ? q : yfin;
-    else return (lane == 0) ? q : yfin;
+__device__ __forceinline__ bool finite64(double v) {
+    return (__double2hiint(v) & 0x7ff00000) != 0x7ff00000;
 }
 
 // partsize 64: full partitions whose slice is at most WMAX wide and whose
@@ -1017,10 +1076,14 @@ __global__ __launch_bounds__(64) void k_sweep_view_ring(const double *x, double 
     const double rr = 1.0 / dr;
     const int64_t e0 = A.sptr[k];
     const int w = (int)((A.sptr[k + 1] - e0) >> 6);
-    double yr;
+    double yr = 0.0;
+    bool ring_done = false, nonfinite = false;
     if (Pa == 64 && w <= WMAX && !__any(!safe_divisor(dr))) {
         yr = view_ring<MODE, WMAX>(xr, dr, rr, lane, s, A, e0, w, ring);
-    } else {
+        nonfinite = __any(!finite64(yr));
+        ring_done = !nonfinite;
+    }
+    if (!ring_done) {
         int64_t plo = 0, pmid = 0, pmid1 = 0, phi = 0;
         if (live) {
             const int64_t p0 = ptr[g] - 1, p1 = ptr[g + 1] - 1;
@@ -1029,7 +1092,9 @@ __global__ __launch_bounds__(64) void k_sweep_view_ring(const double *x, double 
             pmid1 = (pmid < p1 && ind[pmid] - 1 == g) ? pmid + 1 : pmid;
             phi = row_lower_bound(ind, pmid1, p1, s + Pa);
         }
-        if (__any(!safe_divisor(dr)))
+        // true divisions where a diagonal is unsafe, or where a quotient is
+        // not finite (the fast quotient of an infinite x - sum is a NaN)
+        if (nonfinite || __any(!safe_divisor(dr)))
             yr = view_wave<MODE, false>(xr, dr, rr, lane, Pa, s, plo, pmid, pmid1, phi, ind, val);
         else
             yr = view_wave<MODE, true>(xr, dr, rr, lane, Pa, s, plo, pmid, pmid1, phi, ind, val);
@@ -1060,9 +1125,12 @@ __global__ __launch_bounds__(64) void k_sweep_view_ring(const double *x, double 
 // widest window of the wave).
 // A lane only reads and writes its own ring columns.  Slots without an entry
 // hold zero and finished rows take dead additions, as in the partsize-64 ring:
-// the results equal PackedBufferCache's bit for bit.  A wave with a row of
-// more than KS in-partition entries returns false (the caller takes the
-// queue form).
+// the results equal PackedBufferCache's bit for bit, and each row's quotient
+// is formed once after its pass (the additions 0 * y_j leave a finished row's
+// sum as it was while every y_j is finite).  Returns 0 when done; 1 for a wave
+// with a row of more than KS in-partition entries and 2 for a wave with a
+// non-finite result (the caller then takes the queue form on the original x,
+// with true divisions for the latter).
 namespace viewwide {
 constexpr int G = 8;                // steps per group
 constexpr int LG = 3;               // log2 G
@@ -1074,7 +1142,7 @@ constexpr int ring_doubles() { return (R * G + 1) * 64; }  // + one trash slot p
 }  // namespace viewwide
 
 template <int R, int MODE>
-__device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&dr)[R],
+__device__ __forceinline__ int view_ring_wide(double (&xr)[R], const double (&dr)[R],
                                                const double (&rr)[R], int lane, int Pa, int64_t s,
                                                const SellView<double> &A, double *ring) {
     using namespace viewwide;
@@ -1121,7 +1189,7 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
             v[r][q] = t >= 0 ? val : 0.0;
         }
     }
-    if (__any(cnt > K)) return false;
+    if (__any(cnt > K)) return 1;
     // the widest window in the wave bounds the ring-fill scans (wave-uniform)
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off));
@@ -1145,14 +1213,12 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
         }
     };
     RowAcc<double> row[R];
-    double yfin[R];
     auto pass = [&](auto Upper) {
         constexpr bool UP = decltype(Upper)::value;
         constexpr int TAG = UP ? UPPER : 0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             row[r] = RowAcc<double>(xr[r]);
-            yfin[r] = 0.0;
 #pragma unroll
             for (int u = 0; u < G; ++u) mine[64 * (r * G + u)] = 0.0;
         }
@@ -1177,14 +1243,14 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
 #pragma unroll
                 for (int u = 0; u < G; ++u) {
                     const int jl = UP ? 63 - (gb * G + u) : gb * G + u;
-                    const double yj = bcast_keep<64>(row[js].q(dr[js], rr[js]), jl, yfin[js]);
+                    const double yj = seg_bcast<64>(row[js].q(dr[js], rr[js]), jl);
 #pragma unroll
                     for (int r = r0; r < r1; ++r) row[r].add(cur[u][r], yj);
                 }
             }
         });
 #pragma unroll
-        for (int r = 0; r < R; ++r) xr[r] = yfin[r];
+        for (int r = 0; r < R; ++r) xr[r] = row[r].q(dr[r], rr[r]);
     };
     if constexpr (MODE != MODE_BWD) pass(std::false_type{});
     if constexpr (MODE == MODE_SYM) {
@@ -1193,7 +1259,10 @@ __device__ __forceinline__ bool view_ring_wide(double (&xr)[R], const double (&d
         for (int r = 0; r < R; ++r) xr[r] = (lane + 64 * r < Pa) ? dr[r] * xr[r] : 0.0;
     }
     if constexpr (MODE != MODE_FWD) pass(std::true_type{});
-    return true;
+    bool fin = true;
+#pragma unroll
+    for (int r = 0; r < R; ++r) fin = fin && finite64(xr[r]);
+    return __any(!fin) ? 2 : 0;
 }
 
 // view_wave_wide: the general form (any row length, unsafe diagonals) on the
@@ -1271,10 +1340,10 @@ __device__ __forceinline__ void view_wave_wide(double (&xr)[R], const double (&d
     if constexpr (MODE != MODE_FWD) pass(std::true_type{});
 }
 
-// RING: waves whose diagonals are safe take the ring; the others, and waves
-// with a row of too many in-partition entries, the queue form (true
-// divisions where a diagonal is unsafe; both round correctly, so the results
-// are the same).
+// RING: waves whose diagonals are safe take the ring; the others, waves with
+// a row of too many in-partition entries and waves with a non-finite result,
+// the queue form (true divisions where a diagonal is unsafe or a result is
+// not finite; both round correctly, so the results are the same).
 template <int R, int MODE, bool RING>
 __global__ __launch_bounds__(64) void k_sweep_view_wide(const double *x, double *y,
                                                         SellView<double> A,
@@ -1302,12 +1371,17 @@ __global__ __launch_bounds__(64) void k_sweep_view_wide(const double *x, double 
         rr[r] = 1.0 / dr[r];
         unsafe = unsafe || !safe_divisor(dr[r]);
     }
-    bool done_ring = false;
+    int ring_status = 1;
     if constexpr (RING) {
-        if (!__any(unsafe)) done_ring = view_ring_wide<R, MODE>(xr, dr, rr, lane, Pa, s, A, ring);
+        if (!__any(unsafe)) ring_status = view_ring_wide<R, MODE>(xr, dr, rr, lane, Pa, s, A, ring);
     }
-    if (!done_ring) {
-        if (__any(unsafe))
+    if (ring_status != 0) {
+        if constexpr (RING) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) xr[r] = (lane + 64 * r < Pa) ? x[s + lane + 64 * r] : 0.0;
+        }
+        // true divisions where a diagonal is unsafe or a quotient not finite
+        if (ring_status == 2 || __any(unsafe))
             view_wave_wide<R, MODE, false>(xr, dr, rr, lane, Pa, s, ptr, ind, val);
         else
             view_wave_wide<R, MODE, true>(xr, dr, rr, lane, Pa, s, ptr, ind, val);

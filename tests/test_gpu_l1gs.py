@@ -288,6 +288,87 @@ def test_view_wave_equals_packed(gpu, kind, partsize, density):
         np.testing.assert_array_equal(hs.backslash(Pv, x), hs.backslash(Pp, x))
 
 
+def _banded_sym(n, b, seed, keep=0.8):
+    """Symmetric band matrix of half-bandwidth b, each off-diagonal entry kept
+    with probability `keep`: row i has at most 2 b entries inside its
+    partition, and the counts vary from row to row."""
+    rng = np.random.default_rng(seed)
+    R, C, V = [], [], []
+    for k in range(1, b + 1):
+        i = np.arange(n - k)
+        m = rng.random(n - k) < keep
+        R.append(i[m] + k)
+        C.append(i[m])
+        V.append(rng.uniform(-1, 0, m.sum()))
+    L = sp.csc_matrix((np.concatenate(V), (np.concatenate(R), np.concatenate(C))), shape=(n, n))
+    A = L + L.T
+    A = A + sp.diags(np.abs(A).sum(axis=1).A1 + 1.0 + rng.random(n))
+    A = A.tocsc()
+    A.sort_indices()
+    return A
+
+
+@pytest.mark.parametrize("band", [1, 2, 3, 5, 10])
+def test_view_ring_entry_slots(gpu, band):
+    """MatrixViewCache at partsize 64 holds each lane's in-partition entries
+    in 2, 4 or (8, 16, 32 by slice width) slots, the fewest that fit the
+    wave: half-bandwidths 1, 2, 3, 5, 10 put at most 2, 4, 6, 10, 20 entries
+    in a row's partition (slices 3, 5, 7, 11, 21 wide), and rows kept at 80 %
+    mix counts within a wave.  Bit-equal to the packed cache, and to the
+    oracle's MatrixViewCache (forward bit-equal, the others within 1e-12)."""
+    import oracle as O
+    hs = gpu
+    n = 64 * 10 + 17  # ragged last partition: the general form
+    A = _banded_sym(n, band, band)
+    x = np.random.default_rng(band).uniform(-1, 1, n)
+    b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=64, blocks=7)
+    print(f"[view ring slots] band={band}", flush=True)
+    for sw in SWEEPS:
+        Pp = b(A, 64, isSymA=True, sweep=_hs_sweep(hs, sw))
+        Pv = b(A, 64, isSymA=True, sweep=_hs_sweep(hs, sw), cache_strategy=hs.MatrixViewCache())
+        yv = hs.backslash(Pv, x)
+        np.testing.assert_array_equal(yv, hs.backslash(Pp, x), err_msg=sw)
+        ref = O.l1gs_build(A, 64, is_sym=True, sweep=sw, cache="view", gpu=(64, 7),
+                           fast_nonsym=False)
+        yo = O.l1gs_ldiv(ref, x)
+        if sw == "forward":
+            np.testing.assert_array_equal(yv, yo)
+        else:
+            assert relerr(yv, yo) < TOL, (sw, relerr(yv, yo))
+
+
+@pytest.mark.parametrize("partsize", [64, 128])
+def test_view_nonfinite_input(gpu, partsize):
+    """Infinite and NaN inputs through the MatrixViewCache rings: a wave whose
+    result is not finite is redone by rows with true divisions, so the output
+    follows the reference's MatrixViewCache arithmetic -- only stored entries
+    are added (no 0 * inf from the ring's empty slots) and an infinite
+    numerator divides to +-inf -- and matches the oracle's view sweep: the same
+    NaN and infinite entries, the finite ones within 1e-12."""
+    import oracle as O
+    hs = gpu
+    n = 64 * 10 + 17
+    A = _banded_sym(n, 2, 1)
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    x[70], x[300], x[500] = np.inf, -np.inf, np.nan
+    b = hs.L1GSPrecBuilder(hs.ROCBackend(), threads=64, blocks=7)
+    print(f"[view nonfinite] P={partsize}", flush=True)
+    for sw in SWEEPS:
+        Pv = b(A, partsize, isSymA=True, sweep=_hs_sweep(hs, sw),
+               cache_strategy=hs.MatrixViewCache())
+        yv = hs.backslash(Pv, x)
+        ref = O.l1gs_build(A, partsize, is_sym=True, sweep=sw, cache="view", gpu=(64, 7),
+                           fast_nonsym=False)
+        yo = O.l1gs_ldiv(ref, x)
+        assert np.isinf(yo).any() and np.isnan(yo).any()
+        np.testing.assert_array_equal(np.isnan(yv), np.isnan(yo), err_msg=sw)
+        inf = np.isinf(yo)
+        np.testing.assert_array_equal(yv[inf], yo[inf], err_msg=sw)
+        fin = np.isfinite(yo)
+        assert np.isfinite(yv[fin]).all(), sw
+        assert relerr(yv[fin], yo[fin]) < TOL, (sw, relerr(yv[fin], yo[fin]))
+
+
 @pytest.mark.parametrize("kind", ["sym_csc", "csr", "nonsym_csc"])
 @pytest.mark.parametrize("partsize,density", [(65, 0.004), (100, 0.006), (128, 0.004),
                                               (128, 0.009), (128, 0.03), (160, 0.004),
