@@ -425,22 +425,24 @@ __global__ __launch_bounds__(256) void k_sweep_seg(const double *x, double *y,
 // recurrence reading its columns from LDS.  Loading columns straight from
 // memory issues one partly overlapping wave load per step; this issues
 // (64/PM) P(P-1)/2 / 64 aligned ones.  A wave holding the last (short)
-// partition keeps the direct path.
-template <int PM, int MODE, class TS, class TC>
-__global__ __launch_bounds__(256) void k_sweep_seg_lds(const double *x, double *y,
-                                                       const TS *__restrict__ T,
-                                                       const TC *__restrict__ D, int64_t n, int P,
-                                                       int64_t nparts, const int *__restrict__ done) {
+// partition keeps the direct path.  WPB waves per workgroup, each with its own
+// LDS area and no workgroup barrier.
+template <int PM, int MODE, class TS, class TC, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_sweep_seg_lds(const double *x, double *y,
+                                                            const TS *__restrict__ T,
+                                                            const TC *__restrict__ D, int64_t n,
+                                                            int P, int64_t nparts,
+                                                            const int *__restrict__ done) {
     static_assert(MODE == MODE_FWD || MODE == MODE_BWD, "one triangle per block");
     if (done && *done) return;
     constexpr int SW = 64 / PM;             // partitions per wave
     constexpr int BLK = PM * (PM - 1) / 2;  // elements per block
     constexpr int WE = SW * BLK;            // elements per wave
     constexpr int NCH = (WE + 63) / 64;
-    __shared__ TS lds_all[4 * NCH * 64];
+    __shared__ TS lds_all[WPB * NCH * 64];
     const int wave = threadIdx.x >> 6, l64 = threadIdx.x & 63;
     const int lane = l64 % PM, seg = l64 / PM;
-    const int64_t kw = ((int64_t)blockIdx.x * 4 + wave) * SW;  // the wave's first partition
+    const int64_t kw = ((int64_t)blockIdx.x * WPB + wave) * SW;  // the wave's first partition
     const int64_t k = kw + seg;
     if (__all(k >= nparts)) return;
     const bool staged = (kw + SW) * (int64_t)PM <= n;  // the same in every lane of the wave
@@ -1438,12 +1440,17 @@ static void launch_seg(const L1GSPlan &h, const double *x, double *y, const TC *
     int P = (int)h.P;
     if constexpr (PM == 16 || PM == 32) {
         if (P == PM && h.sweep != HS_SWEEP_SYMMETRIC) {
+            // one wave per workgroup: a wave stages 8 KB (FP64); workgroups
+            // of four (32 KB) fit four to a CU, 16 waves, single waves
+            // 16 to 19 (2048^2 forward / backward 118 -> 108 us)
+            constexpr int WPB = 1;
+            const unsigned g1 = grid_for(h.nparts, WPB * 64 / PM);
             if (h.sweep == HS_SWEEP_FORWARD)
-                hipLaunchKernelGGL((k_sweep_seg_lds<PM, MODE_FWD, TS, TC>), dim3(grid), dim3(256),
-                                   0, st, x, y, T, D, h.n, P, h.nparts, done);
+                hipLaunchKernelGGL((k_sweep_seg_lds<PM, MODE_FWD, TS, TC, WPB>), dim3(g1),
+                                   dim3(64 * WPB), 0, st, x, y, T, D, h.n, P, h.nparts, done);
             else
-                hipLaunchKernelGGL((k_sweep_seg_lds<PM, MODE_BWD, TS, TC>), dim3(grid), dim3(256),
-                                   0, st, x, y, T, D, h.n, P, h.nparts, done);
+                hipLaunchKernelGGL((k_sweep_seg_lds<PM, MODE_BWD, TS, TC, WPB>), dim3(g1),
+                                   dim3(64 * WPB), 0, st, x, y, T, D, h.n, P, h.nparts, done);
             return;
         }
     }
