@@ -339,6 +339,15 @@ __device__ __forceinline__ TC sweep_ring64(TC xr, TC dr, TC rr, int lane,
     return (lane == 0) ? q : yfin;
 }
 
+// LDS elements allocated per workgroup: the ring uses NR * 64 of them; the
+// rest holds a CU to 8 workgroups in FP64 (20 KB each), 16 in FP32.  Fewer
+// block streams at once run faster: the 128^3 FP64 apply took 176-178 us with
+// the 8 KB ring alone (19 workgroups per CU), 171 us at 8 per CU, 172 us at
+// 9-11, 179 us at 6 and 217 us at 4; FP32 92.5 -> 90.3 us.  The fused
+// smoother form, whose prologue gathers x for the residual, keeps the ring
+// alone (its round took 0.40 ms so, 0.42 ms at 8 per CU).
+constexpr int RING64_ALLOC = 2560;
+
 template <class TS, class TC, int NR, int RUN, int FORM>
 __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
                                                      const TS *__restrict__ T,
@@ -346,7 +355,8 @@ __global__ __launch_bounds__(64) void k_sweep_ring64(const double *x, double *y,
                                                      int64_t nparts, const int *__restrict__ done,
                                                      CGFuse f, SmoothFuse sf = SmoothFuse{}) {
     if (done && *done) return;
-    __shared__ TS ring[NR * 64];
+    static_assert(NR * 64 <= RING64_ALLOC, "the ring fits its allocation");
+    __shared__ TS ring[FORM == FORM_SMOOTH ? NR * 64 : RING64_ALLOC];
     const int lane = threadIdx.x;
     const int64_t k = blockIdx.x;
     if (k >= nparts) return;
