@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_b_sweep(const Tile *__restrict__ tiles,
     __shared__ double sm[4];
     constexpr int SEGS = TILE / PM;
     constexpr int MODE = SYMT ? MODE_SYMT : MODE_SYM;
-    __shared__ TS lds_all[SYMT ? SEGS * symt_lds(PM) : 1];
+    __shared__ __align__(16) TS lds_all[SYMT ? SEGS * symt_lds(PM) : 1];  // staged word stores
     TS *lds = SYMT ? lds_all + (threadIdx.x / PM) * symt_lds(PM) : nullptr;
     int t0, t1;
     tile_range(cnt->ntiles, t0, t1);

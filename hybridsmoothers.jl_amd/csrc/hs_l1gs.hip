@@ -502,7 +502,7 @@ __global__ __launch_bounds__(64) void k_sweep_symt(const double *x, double *y,
                                                    SmoothFuse sf = SmoothFuse{}) {
     if (done && *done) return;
     constexpr int SEGS = 64 / PM;
-    __shared__ TS lds[SEGS * symt_lds(PM)];
+    __shared__ __align__(16) TS lds[SEGS * symt_lds(PM)];  // word stores of the staged form
     const int lane = threadIdx.x % PM;
     const int seg = threadIdx.x / PM;
     const int64_t k = (int64_t)blockIdx.x * SEGS + seg;

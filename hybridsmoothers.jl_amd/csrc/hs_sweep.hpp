@@ -564,12 +564,12 @@ struct SymtStage {
     static constexpr int SEGS = 64 / PM;
     static constexpr int BLK = PM * (PM - 1) / 2;
     static constexpr int WE = SEGS * BLK;
-    static constexpr int NCH = (WE + 63) / 64;
     static constexpr int colL(int j) { return j * (PM - 1) - j * (j - 1) / 2; }
-    // chunks that must be in LDS before forward column j is read (all segments)
-    static constexpr int need(int j) {
+    // chunks of cw values that must be in LDS before forward column j is read
+    // (all segments)
+    static constexpr int need(int j, int cw) {
         const int hi = (SEGS - 1) * BLK + colL(j + 1) - 1;  // highest value read
-        return hi / 64 + 1;
+        return hi / cw + 1;
     }
 };
 
@@ -577,22 +577,34 @@ template <int PM, class TS, class TC>
 __device__ __forceinline__ TC sweep_symt_staged(TC xr, TC dr, TC rr, int lane, int seg,
                                                 const TS *__restrict__ wsrc, TS *wl) {
     using St = SymtStage<PM>;
-    constexpr int NL = PM - 1, NCH = St::NCH, WE = St::WE, BLK = St::BLK;
+    constexpr int NL = PM - 1, WE = St::WE, BLK = St::BLK;
+    // each lane loads V consecutive values as one word W: BF16 four to an
+    // 8-byte word, so that a load moves 512 B per wave as in FP64 (the 2-byte
+    // loads took the 128^3 BF16 apply 46-47 us, 8-byte ones 44; FP32 measured
+    // slower with 8-byte words and keeps one value per lane).  A wave's
+    // blocks start word-aligned: WE and the wave's first block offset are
+    // multiples of V, the buffers 256-byte aligned.
+    using W = std::conditional_t<sizeof(TS) == 2, uint64_t,
+                                 std::conditional_t<sizeof(TS) == 4, uint32_t, uint64_t>>;
+    constexpr int V = (int)(sizeof(W) / sizeof(TS)), CW = 64 * V, NCH = (WE + CW - 1) / CW;
+    static_assert(WE % V == 0, "a wave's blocks are whole words");
     constexpr int RB = NL < 4 ? NL : 4;  // reads in flight, at most one per column
-    static_assert(St::need(NL - 1) == NCH, "the forward pass reads every chunk");
+    static_assert(St::need(NL - 1, CW) == NCH, "the forward pass reads every chunk");
     const int l64 = (int)(threadIdx.x & 63);
-    TS chunk[NCH];
+    W chunk[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        const int idx = 64 * c + l64;
-        chunk[c] = ld_stream(wsrc + (idx < WE ? idx : WE - 1));
+        const int idx = CW * c + V * l64;
+        chunk[c] = __builtin_nontemporal_load(
+            reinterpret_cast<const W *>(wsrc + (idx < WE ? idx : WE - V)));
     }
     auto lds_fence = [] { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); };
     // chunks [A, B) to LDS (compile-time range)
     auto write_chunks = [&](auto A, auto B) {
         static_for<decltype(A)::value, decltype(B)::value>([&](auto C) {
             constexpr int c = decltype(C)::value;
-            if (64 * c + l64 < WE) wl[64 * c + l64] = chunk[c];
+            const int idx = CW * c + V * l64;
+            if (idx < WE) *reinterpret_cast<W *>(wl + idx) = chunk[c];
         });
     };
     // per-lane bases, opaque so that per-step addresses fold into immediate offsets
@@ -607,7 +619,7 @@ __device__ __forceinline__ TC sweep_symt_staged(TC xr, TC dr, TC rr, int lane, i
     };
     const int base4 = seg_base4(PM);
 
-    write_chunks(std::integral_constant<int, 0>{}, std::integral_constant<int, St::need(RB - 1)>{});
+    write_chunks(std::integral_constant<int, 0>{}, std::integral_constant<int, St::need(RB - 1, CW)>{});
     lds_fence();
     TS rb[RB];
     static_for<0, RB>([&](auto C) { rb[decltype(C)::value] = fwd_read(C); });
@@ -618,7 +630,7 @@ __device__ __forceinline__ TC sweep_symt_staged(TC xr, TC dr, TC rr, int lane, i
         __builtin_amdgcn_sched_barrier(0);
         const TS v = rb[j % RB];
         if constexpr (j + RB < NL) {
-            constexpr int a = St::need(j + RB - 1), b = St::need(j + RB);
+            constexpr int a = St::need(j + RB - 1, CW), b = St::need(j + RB, CW);
             if constexpr (b > a) {
                 write_chunks(std::integral_constant<int, a>{}, std::integral_constant<int, b>{});
                 lds_fence();
