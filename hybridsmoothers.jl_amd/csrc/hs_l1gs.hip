@@ -502,7 +502,12 @@ __global__ __launch_bounds__(64) void k_sweep_symt(const double *x, double *y,
                                                    SmoothFuse sf = SmoothFuse{}) {
     if (done && *done) return;
     constexpr int SEGS = 64 / PM;
-    __shared__ __align__(16) TS lds[SEGS * symt_lds(PM)];  // word stores of the staged form
+    // PM = 64: one partition per wave, so a wave either stages its full block
+    // or holds the short last partition (register form); LDS holds only the
+    // block, 2,016 values (measured within run-to-run spread of the larger
+    // 2,144-value area, which the staged form does not use)
+    constexpr int LSTR = PM == 64 ? PM * (PM - 1) / 2 : symt_lds(PM);
+    __shared__ __align__(16) TS lds[SEGS * LSTR];  // word stores of the staged form
     const int lane = threadIdx.x % PM;
     const int seg = threadIdx.x / PM;
     const int64_t k = (int64_t)blockIdx.x * SEGS + seg;
@@ -511,8 +516,8 @@ __global__ __launch_bounds__(64) void k_sweep_symt(const double *x, double *y,
             if (threadIdx.x == 0) f.partials[blockIdx.x] = 0.0;
         return;
     }
-    double v = sweep_seg_body<PM, MODE_SYMT, TS, TC, FORM>(x, y, T, D, n, P, nparts, k, lane, f,
-                                                           sf, lds + seg * symt_lds(PM));
+    double v = sweep_seg_body<PM, MODE_SYMT, TS, TC, FORM, LSTR>(x, y, T, D, n, P, nparts, k,
+                                                                 lane, f, sf, lds + seg * LSTR);
     if constexpr (FORM == FORM_CG) {
         v = wave_sum(v);  // r'z partial of this workgroup
         if (threadIdx.x == 0) f.partials[blockIdx.x] = v;

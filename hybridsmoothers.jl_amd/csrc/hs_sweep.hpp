@@ -676,7 +676,11 @@ __device__ __forceinline__ TC sweep_symt_staged(TC xr, TC dr, TC rr, int lane, i
 // terms to the real rows, whose results are therefore those of the short
 // partition's own recurrence.
 // `output(i, y_i)` stores the lane's result (sweep_seg_run: y[i] = y_i).
-template <int PM, int MODE, class TS, class TC, bool PAD, class In, class Out>
+// LSTR: LDS elements per segment.  With symt_lds(PM) every MODE_SYMT form
+// fits; with the block size PM (PM - 1) / 2 only the staged one, and a wave
+// that cannot stage its blocks takes the register form (same results).
+template <int PM, int MODE, class TS, class TC, bool PAD, int LSTR = symt_lds(PM), class In,
+          class Out>
 __device__ __forceinline__ double sweep_seg_run_io(In &&input, Out &&output,
                                                    const TS *__restrict__ T,
                                                    const TC *__restrict__ D, int64_t n, int P,
@@ -710,9 +714,9 @@ __device__ __forceinline__ double sweep_seg_run_io(In &&input, Out &&output,
         if constexpr (MODE == MODE_SYMT) {
             const int sw = seg_of_wave<PM>();
             yr = sweep_symt_staged<PM, TS, TC>(xr, dr, rr, lane, sw, T + (kk - sw) * SymtStage<PM>::BLK,
-                                               lds - sw * symt_lds(PM));
+                                               lds - sw * LSTR);
         }
-    } else if (__all(Pa == PM)) {
+    } else if (__all(Pa == PM) && (MODE != MODE_SYMT || LSTR >= symt_lds(PM))) {
         if constexpr (MODE == MODE_SYMT && PM == 64)
             yr = sweep_full_symt64<32, TS, TC>(xr, dr, rr, lane, Lk, lds);
         else if constexpr (MODE == MODE_SYMT)
@@ -740,7 +744,7 @@ enum { FORM_PLAIN = 0, FORM_CG = 1, FORM_SMOOTH = 2 };
 
 // The plain form (input x), the fused CG form (input r - alpha q, with the
 // CG vector updates of cg_prologue) and the fused smoother form.
-template <int PM, int MODE, class TS, class TC, int FORM>
+template <int PM, int MODE, class TS, class TC, int FORM, int LSTR = symt_lds(PM)>
 __device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
                                                  const TS *__restrict__ T,
                                                  const TC *__restrict__ D, int64_t n, int P,
@@ -748,7 +752,7 @@ __device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
                                                  const CGFuse &f, const SmoothFuse &sf,
                                                  TS *lds = nullptr) {
     if constexpr (FORM == FORM_CG)
-        return sweep_seg_run_io<PM, MODE, TS, TC, false>(
+        return sweep_seg_run_io<PM, MODE, TS, TC, false, LSTR>(
             [&](int64_t i) { return cg_prologue(f, i); },
             [&](int64_t i, double z) {
                 if constexpr (std::is_same_v<TC, float>) {
@@ -761,13 +765,14 @@ __device__ __forceinline__ double sweep_seg_body(const double *x, double *y,
             },
             T, D, n, P, nparts, k, lane, lds);
     else if constexpr (FORM == FORM_SMOOTH)
-        return sweep_seg_run_io<PM, MODE, TS, TC, false>(
+        return sweep_seg_run_io<PM, MODE, TS, TC, false, LSTR>(
             [&](int64_t i) { return row_dot(sf.A, sf.x, i) - sf.b[i]; },
             [&](int64_t i, double z) { sf.xout[i] = sf.x[i] - z; }, T, D, n, P, nparts, k, lane,
             lds);
     else
-        return sweep_seg_run<PM, MODE, TS, TC, false>([&](int64_t i) { return x[i]; }, y, T, D, n,
-                                                      P, nparts, k, lane, lds);
+        return sweep_seg_run_io<PM, MODE, TS, TC, false, LSTR>(
+            [&](int64_t i) { return x[i]; }, [&](int64_t i, double v) { y[i] = v; }, T, D, n, P,
+            nparts, k, lane, lds);
 }
 
 }  // namespace hs
