@@ -491,8 +491,9 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_seg_lds(const double *x, dou
 
 // One-triangle symmetric sweep (HS_CACHE_PACKED_SYM): one wave per workgroup,
 // 64 / PM partitions per wave, each with its LDS copy of L (symt_lds(PM)
-// elements: 17 KB per wave in FP64 at PM = 64), so that LDS does not limit
-// the waves per CU below what the recurrence's latency needs.
+// elements; at PM = 64 the staged block alone, 16,128 B per wave in FP64), so
+// that LDS does not limit the waves per CU below what the recurrence's
+// latency needs.
 template <int PM, class TS, class TC, int FORM = FORM_PLAIN>
 __global__ __launch_bounds__(64) void k_sweep_symt(const double *x, double *y,
                                                    const TS *__restrict__ T,
