@@ -560,11 +560,12 @@ def main():
                     lambda: hs.ldiv_(y, pv, x, sync=False),
                     (n + 1) * 8 + int(A.nnz) * 16 + 3 * n * 8)
         del pv
-        # partsize 128: one wave per partition, two rows per lane
-        pw = builder(A, 128, isSymA=True, sweep=hs.SymmetricSweep())
-        add_variant("128^3 P=128 symmetric fp64", lambda: hs.ldiv_(y, pw, x, sync=False),
-                    sweep_bytes(n, pw.packed_size))
-        del pw
+        # partsizes 128, 192, 256: one wave per partition, two to four rows per lane
+        for pw_size in (128, 192, 256):
+            pw = builder(A, pw_size, isSymA=True, sweep=hs.SymmetricSweep())
+            add_variant(f"128^3 P={pw_size} symmetric fp64", lambda: hs.ldiv_(y, pw, x, sync=False),
+                        sweep_bytes(n, pw.packed_size))
+            del pw
         # MatrixViewCache at partsize 128 and 256 (R = 2 and 4 rows per lane)
         for pv_size in (128, 256):
             pv = builder(A, pv_size, isSymA=True, sweep=hs.SymmetricSweep(),

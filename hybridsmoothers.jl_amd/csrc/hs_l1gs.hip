@@ -605,6 +605,83 @@ __device__ __forceinline__ void sweep_wide(TC (&xr)[R], const TC (&dr)[R], const
     if constexpr (MODE != MODE_FWD) pass(std::true_type{});
 }
 
+// sweep_wide for a full partition (64 R rows): the steps run in R blocks of
+// 64 whose owner row slot js is known at compile time, so the quotient is the
+// owner slot's, slots already finished take no work, and slots past the
+// owner (forward; before it, backward) have a term in every lane with no
+// test.  In the owner slot the lanes whose rows are finished add a term of
+// the packed block's element 0 (their quotients were kept when they formed:
+// bcast_keep), as in the 64-row kernels.  Loads, products and their order
+// per row are those of sweep_wide, so the results are the same bit for bit.
+template <int R, int MODE, bool FAST, class TS, class TC>
+__device__ __forceinline__ void sweep_wide_full(TC (&xr)[R], const TC (&dr)[R],
+                                                const TC (&rr)[R], int lane,
+                                                const TS *__restrict__ Lk,
+                                                const TS *__restrict__ Uk) {
+    constexpr int P = 64 * R;
+    constexpr int U = 4;                    // steps per group, a divisor of 64
+    constexpr int NGB = 64 / U;             // groups per block
+    static_assert(NGB * U == 64, "the groups cover the block");
+    auto pass = [&](auto Upper) {
+        constexpr bool UP = decltype(Upper)::value;
+        const TS *__restrict__ B = UP ? Uk : Lk;
+        RowAcc<TC> row[R];
+        TC yfin[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            row[r] = RowAcc<TC>(xr[r]);
+            yfin[r] = (TC)0;
+        }
+        static_for<0, R>([&](auto Bk) {
+            constexpr int b = decltype(Bk)::value;
+            constexpr int js = UP ? R - 1 - b : b;  // row slot owning the block's steps
+            constexpr int r0 = UP ? 0 : js, r1 = UP ? js + 1 : R;  // slots with terms
+            auto load_group = [&](int g, TS (&buf)[U][R]) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int jl = UP ? 63 - (g * U + u) : g * U + u;
+                    const int j = 64 * js + jl;
+#pragma unroll
+                    for (int r = r0; r < r1; ++r) {
+                        const int i = lane + 64 * r;
+                        const bool has = (r != js) || (UP ? lane < jl : lane > jl);
+                        const int64_t off = UP ? colU_off(j, P) + i : colL_off(j, P) + (i - j - 1);
+                        buf[u][r] = B[has ? off : 0];
+                    }
+                }
+            };
+            TS cur[U][R], nxt[U][R];
+            load_group(0, cur);
+            // rolled: unrolled, the blocks' loads are hoisted and the kernel
+            // runs out of registers
+#pragma unroll 1
+            for (int g = 0; g < NGB; ++g) {
+                if (g + 1 < NGB) load_group(g + 1, nxt);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int jl = UP ? 63 - (g * U + u) : g * U + u;
+                    const TC q = row[js].template q<FAST>(dr[js], rr[js]);
+                    const TC yj = bcast_keep<64>(q, jl, yfin[js]);
+#pragma unroll
+                    for (int r = r0; r < r1; ++r) row[r].add(load_as<TC>(cur[u][r]), yj);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int r = r0; r < r1; ++r) cur[u][r] = nxt[u][r];
+            }
+        });
+#pragma unroll
+        for (int r = 0; r < R; ++r) xr[r] = yfin[r];
+    };
+    if constexpr (MODE != MODE_BWD) pass(std::false_type{});
+    if constexpr (MODE == MODE_SYM) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) xr[r] = dr[r] * xr[r];  // y .= D_Dl1 .* y (:897)
+    }
+    if constexpr (MODE != MODE_FWD) pass(std::true_type{});
+}
+
 template <int R, int MODE, class TS, class TC>
 __global__ __launch_bounds__(64) void k_sweep_wide(const double *x, double *y,
                                                    const TS *__restrict__ T,
@@ -631,6 +708,8 @@ __global__ __launch_bounds__(64) void k_sweep_wide(const double *x, double *y,
     }
     if (__any(unsafe))
         sweep_wide<R, MODE, false, TS, TC>(xr, dr, rr, lane, Pa, T + lb, T + ub);
+    else if (Pa == 64 * R)
+        sweep_wide_full<R, MODE, true, TS, TC>(xr, dr, rr, lane, T + lb, T + ub);
     else
         sweep_wide<R, MODE, true, TS, TC>(xr, dr, rr, lane, Pa, T + lb, T + ub);
 #pragma unroll
